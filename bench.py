@@ -1,0 +1,157 @@
+"""Benchmark: primary-ray throughput on the 256^3 BC1 cave at 1920x1080 on one MI355X.
+
+A step is one vx_render launch: 2,073,600 pinhole rays traced through the ESVO
+octree, brick DDA and BC1 decode, writing one 32-byte hit record per ray. The
+scene is built and uploaded before timing; the timed region holds exactly
+--steps launches bracketed by synchronisation, measured with HIP events on the
+stream the kernels are launched on.
+
+Prints ONE JSON line. Besides the required keys it carries:
+  roofline      algorithmic bytes per launch / average launch time against the
+                8 TB/s HBM3E peak; the per-ray byte model is
+                8*iterations + 8*brick visits + 4*DDA voxels + 8*hits + 32 (record out),
+                counted by the kernel's own counting variant on the same frame;
+                traffic is the measured HBM bytes per launch when a rocprofv3 PMC
+                summary for this workload is present under profiles/
+  cpu_baseline  the strict-FP32 C oracle on the host cores, same frame, all cores
+                (median of 5 after a warm-up) -- a reported baseline, not a target
+  parity        the last timed frame compared bit for bit with the oracle's frame
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "cave256_bc1_1080p"
+PMC_SUMMARY = ROOT / "profiles" / "k_render_cave256_1080p_pmc.md"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic() -> float | None:
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    if not PMC_SUMMARY.exists():
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", PMC_SUMMARY.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(orc, built, rays, reps: int = 5):
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 64))
+    scene = orc.OracleScene.from_built(built)
+    orc.trace(scene, rays[:20000], nthreads=threads)  # warm-up
+    times = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out, counts = orc.trace(scene, rays, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": len(rays) / med / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"the full {len(rays)}-ray frame, {threads} threads, median of {reps} after a warm-up"}, out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); rays shard by image tile "
+              f"with no exchange, but the multi-GPU launcher is not part of this build", file=sys.stderr)
+        return 2
+
+    import torch
+    import __graft_entry__ as entry
+
+    vx = entry.load_package()
+    from oracle import oracle as orc
+
+    W, H, N = 1920, 1080, 256
+    t0 = time.perf_counter()
+    grid, cam = vx.gen_cave(N, 2, 0.5)
+    built = vx.build_from_dense(grid, bc1=True)
+    log(f"[bench] scene built in {time.perf_counter() - t0:.1f}s: {built.node_count} descriptors, "
+        f"{built.brick_count} bricks")
+    ctx = vx.Context(0)
+    scene = vx.Scene(ctx, built)
+    stream = torch.cuda.current_stream()
+    dev = torch.device("cuda", 0)
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device=dev)
+
+    def step():
+        vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(cam), W, H, 1e30, hits.data_ptr(), None,
+                                   stream.cuda_stream), "vx_render")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    rays = W * H
+    mrays = rays / (ms_step * 1e-3) / 1e6
+    log(f"[bench] {args.steps} launches: {ms_step:.4f} ms/launch, {mrays:.1f} Mrays/s (wall {wall:.3f}s)")
+
+    # algorithmic bytes from the kernel's counting variant on the same rays
+    cam_rays = vx.camera_rays(cam, W, H, 1e30)
+    _, counters = scene.trace_counters(cam_rays)
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy().astype(np.float64)
+    bytes_launch = 8 * c[1] + 8 * c[2] + 4 * c[3] + 8 * c[4] + 32 * c[0]
+    achieved = bytes_launch / (ms_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                "bytes_per_ray": round(bytes_launch / rays, 2),
+                "per_ray": {"iterations": round(c[1] / rays, 3), "brick_visits": round(c[2] / rays, 3),
+                            "dda_voxels": round(c[3] / rays, 3), "hit_rate": round(c[4] / rays, 4)}}
+
+    result = {
+        "metric": "Mrays/s @1080p, 256^3 BC1 octree; batched & fwd+bwd vs plain; HBM GB/s",
+        "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded fBm cave, seed 2, 50% occupancy)",
+        "config": {"workload": WORKLOAD, "grid": N, "rays_per_step": rays, "width": W, "height": H,
+                   "bricks": built.brick_count, "descriptors": built.node_count, "colour": "bc1"},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle on host cores)...")
+        oray = orc.camera_rays(cam.as_floats(), W, H, 1e30)
+        base, want = cpu_baseline(orc, built, oray)
+        result["cpu_baseline"] = base
+        got = vx.hits_to_numpy(hits)
+        diff = int((got.view(np.uint32).reshape(-1, 8) != want.view(np.uint32).reshape(-1, 8)).any(1).sum())
+        result["parity"] = "bit-exact vs oracle" if diff == 0 else f"{diff} records differ from the oracle"
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,77 @@
+/*
+ * vixen_build.h -- C ABI of the host-side scene builder and the synthetic
+ * scene generators.
+ *
+ * The builder produces exactly the buffers vx_scene_create consumes, in VIXEN's
+ * layout and order:
+ *   - brick ids are Morton ranks of the populated 8^3 bricks
+ *     (libraries/SVO/src/SVORebuild.cpp:334-371, Morton interleave
+ *      libraries/Core/src/MortonEncoding.cpp:18-63: x in bit 0, y bit 1, z bit 2);
+ *   - descriptors are in breadth-first order from the root, each node's
+ *     children contiguous in octant order, leaf (brick) descriptors last
+ *     (SVORebuild.cpp:419-598);
+ *   - a leaf descriptor has validMask = leafMask = 0xFF and word1 = brick id
+ *     (SVORebuild.cpp:397-405);
+ *   - brickData is 512 u32 per brick, linear index x + 8y + 64z
+ *     (libraries/CashSystem/src/VoxelSceneCacher.cpp:551-588);
+ *   - BC1 blocks cover 16 consecutive linear voxels, empty voxels encoded as
+ *     black, all-empty blocks stored as 0 (SVORebuild.cpp:613-731, encoder
+ *     libraries/VoxelData/src/Compression/DXT1Compressor.cpp:38-111);
+ *   - the 256-byte OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it, with
+ *     grid bounds [0,N]^3 in voxel units.
+ * Far pointers: a child block index above 0x7FFF sets farBit and is stored in
+ * word1 bits 0-23 (the contour field, unused on this path).
+ */
+#ifndef VIXEN_BUILD_H
+#define VIXEN_BUILD_H
+
+#include "vixen_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vx_built_scene {
+    uint32_t* esvo_nodes;   /* node_count x 2 */
+    uint64_t node_count;
+    uint32_t* brick_data;   /* brick_count x 512 */
+    uint64_t brick_count;
+    uint64_t* bc1_colors;   /* brick_count x 32, NULL when not requested */
+    uint32_t* brick_coords; /* brick_count: bx | by << 10 | bz << 20 (grid cell of each brick) */
+    uint64_t internal_count; /* descriptors above the leaf level */
+    vx_octree_config config;
+} vx_built_scene;
+
+/* Build from a dense grid grid[x + N*(y + N*z)] of RGBA8 words (0 = empty).
+ * N must be a power of two in [16, 4096]. nthreads <= 0 uses all cores. */
+vx_status vx_build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, int nthreads,
+                              vx_built_scene** out);
+void vx_built_scene_free(vx_built_scene* s);
+
+/* BC1 encode of 16 colours in [0,1]^3 (DXT1Compressor::encodeBlockTyped with
+ * validCount 16). */
+uint64_t vx_bc1_encode_block(const float colors[16][3]);
+
+/* Synthetic scenes (voxel units, world = [0,N]^3). Each fills grid (N^3 words)
+ * and a camera (vx_camera, fov in degrees) for the scene's primary view.
+ *   cornell: N^3 room, 1-voxel walls x=0 red, x=N-1 green, y=0, y=N-1 and
+ *            z=N-1 grey 200, two grey boxes 12^3 and 12x24x12 at seeded spots,
+ *            +-8 seeded noise per channel; camera at the open z=0 face centre
+ *            looking +z, 60 degrees, aspect 1.
+ *   cave:    solid where 4-octave fBm value noise (base frequency 1/32,
+ *            lacunarity 2, gain 0.5) exceeds its (1 - occupancy) quantile;
+ *            colour is a height gradient plus seeded noise; camera at a
+ *            seeded empty cell, seeded look direction, 70 degrees, 16:9.
+ *   urban:   (N/32)^2 lots of pitch 32, building footprints 24-30, heights
+ *            uniform in [64,900] * N/1024, a lot is built with probability
+ *            occupancy; 1-voxel ground; seeded colour per building; camera
+ *            above the skyline looking at a seeded ground point, 70 degrees, 16:9. */
+vx_status vx_gen_cornell(uint32_t N, uint64_t seed, uint32_t* grid, vx_camera* cam);
+vx_status vx_gen_cave(uint32_t N, uint64_t seed, double occupancy, uint32_t* grid, vx_camera* cam);
+vx_status vx_gen_urban(uint32_t N, uint64_t seed, double occupancy, uint32_t* grid, vx_camera* cam);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,201 @@
+/*
+ * vixen_hip.h -- C ABI of the MI355X (gfx950) voxel ray-cast backend.
+ *
+ * The path behind this ABI: ESVO (Laine-Karras) sparse-voxel-octree traversal,
+ * then an 8^3 brick DDA, then BC1 (DXT1) brick colour decode. Scene buffers keep
+ * VIXEN's layout: 64-bit child descriptors (bit fields of
+ * libraries/SVO/include/SVOTypes.h:36-117), 512 u32 per brick of `brickData`
+ * (libraries/CashSystem/src/VoxelSceneCacher.cpp:551-588), 32 BC1 blocks of
+ * 8 bytes per brick (libraries/SVO/src/SVORebuild.cpp:613-731) and the
+ * 256-byte OctreeConfig UBO (libraries/CashSystem/include/VoxelSceneCacher.h:100-133).
+ *
+ * Conventions (mirroring the reference's ISVOStructure contract,
+ * libraries/SVO/include/ISVOStructure.h:30-276):
+ *   - every entry point returns a vx_status; nothing throws across the ABI;
+ *     the text of the last failure on the calling thread is vx_last_error();
+ *   - invalid rays (NaN/Inf, zero direction) produce a miss, never an error
+ *     (libraries/SVO/src/SVOTraversal.cpp:238-259);
+ *   - a context owns one device; scenes own their device copies;
+ *   - trace/render calls are asynchronous on the stream they are given and
+ *     take DEVICE pointers for rays, hits and images (inputs already resident
+ *     in HBM); scene creation takes HOST pointers and copies.
+ */
+#ifndef VIXEN_HIP_H
+#define VIXEN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers: an opaque stream handle.
+ * NULL is the device's default (null) stream. */
+typedef void* vx_stream;
+
+typedef enum vx_status {
+    VX_OK = 0,
+    VX_ERR_INVALID_ARG = 1,
+    VX_ERR_INVALID_SCENE = 2,
+    VX_ERR_HIP = 3,
+    VX_ERR_OUT_OF_MEMORY = 4,
+    VX_ERR_UNSUPPORTED = 5
+} vx_status;
+
+/* ------------------------------------------------------------------------ */
+/* Buffer layouts                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* The 256-byte OctreeConfig uniform block, std140, field for field as
+ * libraries/CashSystem/include/VoxelSceneCacher.h:100-133 (and the shader's
+ * copy at shaders/VoxelRayMarch_Compressed.comp:60-76). Matrices are
+ * column-major as glm stores them. */
+typedef struct vx_octree_config {
+    int32_t esvo_max_scale;      /* always 22 */
+    int32_t user_max_levels;     /* log2(resolution) */
+    int32_t brick_depth_levels;  /* 3 for 8^3 bricks */
+    int32_t brick_size;          /* 8 */
+    int32_t min_esvo_scale;      /* esvo_max_scale - user_max_levels + 1 */
+    int32_t brick_esvo_scale;    /* as the reference fills it (always 20) */
+    int32_t bricks_per_axis;     /* resolution / 8 */
+    int32_t _padding1;
+    float grid_min[3];
+    float _padding2;
+    float grid_max[3];
+    float _padding3;
+    float local_to_world[16];
+    float world_to_local[16];
+    float _padding4[16];
+} vx_octree_config;
+
+/* One ray: the batch form of castRay(origin, direction, tMin, tMax)
+ * (libraries/SVO/include/ISVOStructure.h:122-126). 32 bytes. */
+typedef struct vx_ray {
+    float o[3];
+    float tmin;
+    float d[3];   /* need not be unit length: it is normalised as glm::normalize does */
+    float tmax;
+} vx_ray;
+
+/* Hit record, 32 bytes (replaces ISVOStructure::RayHit, ISVOStructure.h:101-112).
+ *   flags bit 0      : hit
+ *   flags bits 1-2   : entry-face axis (0 x, 1 y, 2 z), valid on a hit
+ *   flags bit 3      : entry-face normal sign is positive
+ *   flags bits 16-31 : ESVO loop iterations executed (hit or miss)
+ *   t                : world-space distance to the entry of the hit voxel's AABB,
+ *                      clamped to the start of the traced segment; +inf on a miss
+ *   leaf_node        : index of the leaf (brick) child descriptor; ~0u on a miss
+ *   brick            : brick id (Morton rank); ~0u on a miss
+ *   voxel            : brick-local linear index x + 8y + 64z; ~0u on a miss
+ *   rgb              : decoded colour in [0,1]; 0 on a miss                     */
+typedef struct vx_hit {
+    uint32_t flags;
+    float t;
+    uint32_t leaf_node;
+    uint32_t brick;
+    uint32_t voxel;
+    float rgb[3];
+} vx_hit;
+
+#define VX_HIT_FLAG_HIT 1u
+#define VX_HIT_AXIS(flags) (((flags) >> 1) & 3u)
+#define VX_HIT_ITERS(flags) ((flags) >> 16)
+
+/* The compute pipeline's 64-byte push constant
+ * (shaders/VoxelRayMarch_Compressed.comp:102-111,
+ *  libraries/RenderGraph/include/Data/CameraData.h:17-27). fov is the
+ * vertical field of view in degrees. */
+typedef struct vx_camera {
+    float pos[3];
+    float time;
+    float dir[3];
+    float fov_deg;
+    float up[3];
+    float aspect;
+    float right[3];
+    int32_t debug_mode;
+} vx_camera;
+
+/* Scene buffers, as VoxelSceneData holds them on the CPU
+ * (libraries/CashSystem/include/VoxelSceneCacher.h:190-256). */
+typedef struct vx_scene_desc {
+    const uint32_t* esvo_nodes;   /* node_count x {word0, word1} */
+    uint64_t node_count;
+    const uint32_t* brick_data;   /* brick_count x 512 u32; != 0 means occupied */
+    uint64_t brick_count;
+    const uint64_t* bc1_colors;   /* brick_count x 32 BC1 blocks, or NULL: colour is
+                                     then brick_data's RGBA8 (R in bits 0-7) */
+    const vx_octree_config* config;
+} vx_scene_desc;
+
+typedef struct vx_ctx vx_ctx;
+typedef struct vx_scene vx_scene;
+
+/* ------------------------------------------------------------------------ */
+/* Context and scenes                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Thread-local text of the last failure on this thread ("" if none). */
+const char* vx_last_error(void);
+
+/* Version string of this build (kernel ISA and options). */
+const char* vx_version(void);
+
+/* Replaces the Vulkan device/queue set-up around ComputeDispatchNode
+ * (libraries/RenderGraph/src/Nodes/ComputeDispatchNode.cpp:134-317). */
+vx_status vx_ctx_create(int device, vx_ctx** out);
+vx_status vx_ctx_destroy(vx_ctx* ctx);
+
+/* Uploads a scene (VoxelSceneCacher::UploadToGPU,
+ * libraries/CashSystem/src/VoxelSceneCacher.cpp:719-891). Validates the tree:
+ * every child pointer in range, leaf bits exactly at the brick-parent level,
+ * brick ids below brick_count. Far pointers: when a descriptor's farBit
+ * (bit 15) is set, its child block index is word1 bits 0-23 instead of the
+ * 15-bit childPointer (the contour field is unused on the voxel path). */
+vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out);
+vx_status vx_scene_destroy(vx_scene* scene);
+
+/* Device addresses of a scene's pools (for callers that share them). */
+vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* brick_count,
+                        int32_t* user_max_levels, int32_t* bricks_per_axis);
+
+/* ------------------------------------------------------------------------ */
+/* Tracing                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Batch castRay (libraries/SVO/src/SVOTraversal.cpp:189-198 -> castRayImpl
+ * :516-691, traversal as shaders/ESVOTraversal.glsl:84-372, leaf addressing as
+ * shaders/VoxelRayMarch_Compressed.comp:283-305, brick DDA and hit semantics as
+ * libraries/SVO/src/SVOBrickDDA.cpp:187-225,390-533, BC1 decode as
+ * libraries/VoxelData/src/Compression/DXT1Compressor.cpp:14-36).
+ * rays and hits are device pointers of n elements. */
+vx_status vx_trace(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits, vx_stream stream);
+
+/* The compute pipeline's per-frame dispatch (VoxelRayMarch_Compressed.comp:493-594):
+ * one pinhole ray per pixel (shaders/RayGeneration.glsl:61-71, pixel centre
+ * (p + 0.5) / size, Vulkan y-flip), traced over [0, tmax]. Writes a w*h hit
+ * record image (row-major, y down) when hits != NULL and/or an RGBA8 image
+ * shaded with computeLighting (shaders/Lighting.glsl:19-28) and the sky
+ * gradient of VoxelRayMarch_Compressed.comp:541-548 when rgba8 != NULL. */
+vx_status vx_render(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
+                    float tmax, vx_hit* hits, uint32_t* rgba8, vx_stream stream);
+
+/* The rays vx_render traces, written out (w*h vx_ray, device pointer). */
+vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t height, float tmax,
+                         vx_ray* rays, vx_stream stream);
+
+/* Per-launch traversal counters for the algorithmic-bytes model:
+ * counters[0] rays, [1] ESVO iterations (8-byte descriptor fetch each),
+ * [2] brick visits (8-byte leaf descriptor fetch each), [3] DDA voxels examined
+ * (4-byte occupancy read each), [4] hits (8-byte BC1 block or 4-byte RGBA8 read).
+ * Runs the same traversal as vx_trace with counting enabled; counters is a
+ * device pointer to 8 u64 that the call zeroes first. */
+vx_status vx_trace_counters(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits,
+                            uint64_t* counters, vx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIXEN_HIP_H */

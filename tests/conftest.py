@@ -1,0 +1,70 @@
+"""Shared fixtures: the package (vbvs--vixen_amd), the oracle and seeded scenes."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+import __graft_entry__ as entry  # noqa: E402
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); runs through the C ABI")
+
+
+@pytest.fixture(scope="session")
+def vx():
+    return entry.load_package()
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def cornell(vx):
+    grid, cam = vx.gen_cornell(64, 1)
+    return grid, cam, vx.build_from_dense(grid, bc1=False)
+
+
+@pytest.fixture(scope="session")
+def cornell_bc1(vx, cornell):
+    grid, cam, _ = cornell
+    return grid, cam, vx.build_from_dense(grid, bc1=True)
+
+
+@pytest.fixture(scope="session")
+def cave(vx):
+    grid, cam = vx.gen_cave(256, 2, 0.5)
+    return grid, cam, vx.build_from_dense(grid, bc1=True)
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx(vx):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return vx.Context(0)
+
+
+def global_voxels(built, hits: np.ndarray) -> np.ndarray:
+    """Hit records -> global voxel coordinates (x, y, z), -1 for misses."""
+    hit = (hits["flags"] & 1) == 1
+    bc = built.brick_coords[np.where(hit, hits["brick"], 0)]
+    lin = hits["voxel"].astype(np.int64)
+    g = np.stack([bc[:, 0] * 8 + (lin & 7), bc[:, 1] * 8 + ((lin >> 3) & 7), bc[:, 2] * 8 + ((lin >> 6) & 7)], 1)
+    return np.where(hit[:, None], g, -1)
+
+
+def record_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Indices of hit records whose 32 bytes differ."""
+    wa = a.view(np.uint32).reshape(-1, 8)
+    wb = b.view(np.uint32).reshape(-1, 8)
+    return np.nonzero((wa != wb).any(1))[0]

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Integer fields (flags incl. iteration count, leaf descriptor, brick, voxel) and
+the FP32 bits of t and rgb must be identical: the comparison is on the raw 32
+bytes of every hit record.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import global_voxels, record_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _assert_same(got, want, rays=None, label=""):
+    bad = record_diff(got, want)
+    if len(bad):
+        i = bad[0]
+        msg = f"{label}: {len(bad)} of {len(want)} hit records differ; first at {i}: got {got[i]} want {want[i]}"
+        if rays is not None:
+            msg += f" ray {rays[i]}"
+        raise AssertionError(msg)
+
+
+def test_camera_rays_bit_exact(vx, orc, cornell, cave, gpu_ctx):
+    for cam, w, h in ((cornell[1], 512, 512), (cave[1], 1920, 1080), (cave[1], 37, 19)):
+        dev = vx.camera_rays(cam, w, h, 1e4)
+        _sync()
+        got = dev.cpu().numpy().view(orc.RAY_DTYPE)
+        want = orc.camera_rays(cam.as_floats(), w, h, 1e4)
+        assert got.tobytes() == want.tobytes(), f"camera rays differ at {w}x{h}"
+
+
+@pytest.mark.parametrize("scene_name", ["cornell", "cornell_bc1"])
+def test_render_cornell_512_bit_exact(vx, orc, scene_name, request, gpu_ctx):
+    grid, cam, built = request.getfixturevalue(scene_name)
+    scene = vx.Scene(gpu_ctx, built)
+    hits, _ = scene.render(cam, 512, 512, tmax=1e4)
+    _sync()
+    got = vx.hits_to_numpy(hits)
+    rays = orc.camera_rays(cam.as_floats(), 512, 512, 1e4)
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    _assert_same(got, want, rays, scene_name)
+    assert (got["flags"] & 1).mean() > 0.99  # a closed room: every primary ray hits
+
+
+def test_render_cave_1080p_bit_exact(vx, orc, cave, gpu_ctx):
+    grid, cam, built = cave
+    scene = vx.Scene(gpu_ctx, built)
+    hits, _ = scene.render(cam, 1920, 1080, tmax=1e30)
+    _sync()
+    got = vx.hits_to_numpy(hits)
+    rays = orc.camera_rays(cam.as_floats(), 1920, 1080, 1e30)
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    _assert_same(got, want, rays, "cave 1080p")
+
+
+def _random_rays(rng, n, N, inside_frac=0.5):
+    r = np.zeros(n, dtype=[("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+    k = int(n * inside_frac)
+    r["o"][:k] = rng.uniform(0, N, (k, 3))
+    r["o"][k:] = rng.uniform(-0.5 * N, 1.5 * N, (n - k, 3))
+    r["d"] = rng.normal(size=(n, 3))
+    r["tmin"] = np.where(rng.random(n) < 0.2, rng.uniform(0, N * 0.3, n), 0.0)
+    r["tmax"] = np.where(rng.random(n) < 0.2, rng.uniform(0, N, n), 1e30)
+    return r
+
+
+def test_trace_random_rays_bit_exact(vx, orc, cave, cornell_bc1, gpu_ctx):
+    rng = np.random.default_rng(7)
+    for grid, cam, built in (cave, cornell_bc1):
+        scene = vx.Scene(gpu_ctx, built)
+        rays = _random_rays(rng, 300_000, built.N)
+        rays = rays.astype(orc.RAY_DTYPE)
+        got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+        _sync()
+        want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+        _assert_same(got, want, rays, f"random rays N={built.N}")
+
+
+def test_trace_edge_rays_bit_exact(vx, orc, cave, gpu_ctx):
+    """Invalid, axis-parallel, grazing, outside and clipped rays."""
+    grid, cam, built = cave
+    N = float(built.N)
+    nan, inf = np.float32(np.nan), np.float32(np.inf)
+    rows = [
+        ((10, 10, 10), (0, 0, 0), 0, 1e30),          # zero direction -> miss
+        ((nan, 1, 1), (1, 0, 0), 0, 1e30),           # NaN origin -> miss
+        ((1, 1, 1), (inf, 0, 0), 0, 1e30),           # infinite direction -> miss
+        ((-5, 100.5, 100.5), (1, 0, 0), 0, 1e30),    # axis-parallel from outside
+        ((100.5, -5, 100.5), (0, 1, 0), 0, 1e30),
+        ((100.5, 100.5, N + 5), (0, 0, -1), 0, 1e30),
+        ((0, 0, 0), (1, 1, 1), 0, 1e30),             # corner diagonal
+        ((-10, 50, 50), (1, 0, 0), 0, 5.0),          # tmax before the volume -> miss
+        ((-10, 50, 50), (1, 0, 0), 500, 1e30),       # tmin after the volume -> miss
+        ((N, 64, 64), (0, 1, 0), 0, 1e30),           # along the max-x face
+        ((0, 64, 64), (0, 0, 1), 0, 1e30),           # along the min-x face
+        ((64, 64, 64), (1e-7, 1, 1e-9), 0, 1e30),    # nearly parallel
+        ((128, 128, 128), (-1, 2e-6, 3e-6), 0, 1e30),
+        ((-100, -100, -100), (-1, -1, -1), 0, 1e30),  # pointing away
+        ((128.0, 128.0, 128.0), (1, 0, 0), 3.0, 3.5),  # short clipped segment
+    ]
+    rays = np.zeros(len(rows), orc.RAY_DTYPE)
+    for i, (o, d, t0, t1) in enumerate(rows):
+        rays[i]["o"], rays[i]["d"], rays[i]["tmin"], rays[i]["tmax"] = o, d, t0, t1
+    scene = vx.Scene(gpu_ctx, built)
+    got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+    _sync()
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays)
+    _assert_same(got, want, rays, "edge rays")
+    assert not (got["flags"][[0, 1, 2, 7, 8, 13]] & 1).any()
+
+
+def test_counters_match_oracle(vx, orc, cave, gpu_ctx):
+    grid, cam, built = cave
+    scene = vx.Scene(gpu_ctx, built)
+    rays = orc.camera_rays(cam.as_floats(), 640, 360, 1e30)
+    hits, counters = scene.trace_counters(vx.rays_to_device(rays))
+    _sync()
+    want, counts = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    _assert_same(vx.hits_to_numpy(hits), want, rays, "counters run")
+    assert counters.cpu().numpy()[:5].tolist() == counts.astype(np.int64).tolist()
+
+
+def test_far_pointer_scene_bit_exact(vx, orc, gpu_ctx):
+    """A dense cave has more than 32767 descriptors, so child blocks use far pointers."""
+    grid, cam = vx.gen_cave(256, 11, 0.9)
+    built = vx.build_from_dense(grid, bc1=True)
+    assert built.node_count > 0x7FFF
+    assert (built.nodes[:, 0] & 0x8000).any()
+    scene = vx.Scene(gpu_ctx, built)
+    rng = np.random.default_rng(3)
+    rays = _random_rays(rng, 200_000, built.N, inside_frac=0.3).astype(orc.RAY_DTYPE)
+    got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+    _sync()
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    _assert_same(got, want, rays, "far pointers")
+    # hits land in voxels that are solid in the dense grid
+    g = global_voxels(built, got)
+    hit = g[:, 0] >= 0
+    assert hit.any()
+    assert (grid[g[hit, 2], g[hit, 1], g[hit, 0]] != 0).all()
+
+
+def test_render_rgba_image(vx, cornell_bc1, gpu_ctx):
+    grid, cam, built = cornell_bc1
+    scene = vx.Scene(gpu_ctx, built)
+    hits, img = scene.render(cam, 128, 128, tmax=1e4, hits=True, rgba=True)
+    _sync()
+    h = vx.hits_to_numpy(hits)
+    px = img.cpu().numpy().view(np.uint32)
+    assert ((px >> 24) == 255).all()
+    hit = (h["flags"] & 1) == 1
+    # shaded hits are the decoded colour times (0.3 + 0.7 max(n.l, 0)), in [0.3, 1] of it
+    r = (px & 255).astype(np.float64) / 255.0
+    want = h["rgb"][:, 0].astype(np.float64)
+    assert np.all(r[hit] <= want[hit] + 1.0 / 255 + 1e-6)
+    assert np.all(r[hit] >= 0.3 * want[hit] - 1.0 / 255 - 1e-6)

@@ -1,0 +1,392 @@
+"""vbvs--vixen_amd: MI355X (gfx950) backend for VIXEN's voxel ray-cast hot path.
+
+Python mirror of the reference's ray-cast surface over the C ABI in
+``libvixen_hip.so`` (declared in ``include/vixen_hip.h`` and
+``include/vixen_build.h``):
+
+* ``build_from_dense`` / ``gen_*`` -- the scene build path
+  (``LaineKarrasOctree::rebuild``, libraries/SVO/src/SVORebuild.cpp:228-732, and
+  ``VoxelSceneCacher::BuildOctree``, libraries/CashSystem/src/VoxelSceneCacher.cpp:463-628);
+* ``Scene.trace`` -- the batch form of ``ISVOStructure::castRay``
+  (libraries/SVO/include/ISVOStructure.h:122-126);
+* ``Scene.render`` -- the compute pipeline's per-frame dispatch
+  (shaders/VoxelRayMarch_Compressed.comp:493-594);
+* ``LaineKarrasOctreeHip.castRay`` -- one-ray convenience with the reference's
+  argument meaning (origin, direction, tMin, tMax) returning a ``RayHit``.
+
+PyTorch is used only for device memory and streams. Every call goes through the
+HIP library; there is no CPU fallback: if the library is missing, importing this
+package raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libvixen_hip.so"
+
+VX_OK = 0
+STATUS_NAMES = {0: "VX_OK", 1: "VX_ERR_INVALID_ARG", 2: "VX_ERR_INVALID_SCENE", 3: "VX_ERR_HIP",
+                4: "VX_ERR_OUT_OF_MEMORY", 5: "VX_ERR_UNSUPPORTED"}
+
+HIT_FLAG = 1
+
+
+class VixenError(RuntimeError):
+    """A C-ABI call returned a non-zero vx_status."""
+
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+# --------------------------------------------------------------------------
+# ABI structures (must match include/vixen_hip.h; sizes are checked at load)
+# --------------------------------------------------------------------------
+class VxOctreeConfig(C.Structure):
+    _fields_ = [("esvo_max_scale", C.c_int32), ("user_max_levels", C.c_int32),
+                ("brick_depth_levels", C.c_int32), ("brick_size", C.c_int32),
+                ("min_esvo_scale", C.c_int32), ("brick_esvo_scale", C.c_int32),
+                ("bricks_per_axis", C.c_int32), ("_padding1", C.c_int32),
+                ("grid_min", C.c_float * 3), ("_padding2", C.c_float),
+                ("grid_max", C.c_float * 3), ("_padding3", C.c_float),
+                ("local_to_world", C.c_float * 16), ("world_to_local", C.c_float * 16),
+                ("_padding4", C.c_float * 16)]
+
+
+class VxCamera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("time", C.c_float), ("dir", C.c_float * 3), ("fov_deg", C.c_float),
+                ("up", C.c_float * 3), ("aspect", C.c_float), ("right", C.c_float * 3),
+                ("debug_mode", C.c_int32)]
+
+    def as_floats(self) -> np.ndarray:
+        """The 64-byte push constant as 16 float32 words (debug_mode bit-cast)."""
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+    def to_dict(self) -> dict:
+        return {k: (list(getattr(self, k)) if k in ("pos", "dir", "up", "right") else getattr(self, k))
+                for k, _ in self._fields_}
+
+
+class VxSceneDesc(C.Structure):
+    _fields_ = [("esvo_nodes", C.c_void_p), ("node_count", C.c_uint64),
+                ("brick_data", C.c_void_p), ("brick_count", C.c_uint64),
+                ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig))]
+
+
+class VxBuiltScene(C.Structure):
+    _fields_ = [("esvo_nodes", C.POINTER(C.c_uint32)), ("node_count", C.c_uint64),
+                ("brick_data", C.POINTER(C.c_uint32)), ("brick_count", C.c_uint64),
+                ("bc1_colors", C.POINTER(C.c_uint64)), ("brick_coords", C.POINTER(C.c_uint32)),
+                ("internal_count", C.c_uint64), ("config", VxOctreeConfig)]
+
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("flags", "<u4"), ("t", "<f4"), ("leaf_node", "<u4"), ("brick", "<u4"),
+                      ("voxel", "<u4"), ("rgb", "<f4", 3)])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
+assert C.sizeof(VxOctreeConfig) == 256 and C.sizeof(VxCamera) == 64
+
+
+def _load() -> C.CDLL:
+    # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Import it first so this library binds to the runtime torch already uses:
+    # one runtime per process, so torch tensors and streams are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() or "
+                          f"python vbvs--vixen_amd/build.py")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u64, u32, i32, st = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_int
+    sig = {
+        "vx_last_error": ([], C.c_char_p),
+        "vx_version": ([], C.c_char_p),
+        "vx_ctx_create": ([C.c_int, C.POINTER(vp)], st),
+        "vx_ctx_destroy": ([vp], st),
+        "vx_scene_create": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
+        "vx_scene_destroy": ([vp], st),
+        "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
+        "vx_trace": ([vp, vp, u64, vp, vp], st),
+        "vx_trace_counters": ([vp, vp, u64, vp, vp, vp], st),
+        "vx_render": ([vp, C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp, vp], st),
+        "vx_camera_rays": ([C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp], st),
+        "vx_build_from_dense": ([vp, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
+        "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
+        "vx_bc1_encode_block": ([vp], u64),
+        "vx_gen_cornell": ([u32, u64, vp, C.POINTER(VxCamera)], st),
+        "vx_gen_cave": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
+        "vx_gen_urban": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+LIB = _load()
+
+# Every symbol the headers declare (the CPU suite checks the library exports them all).
+EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create",
+            "vx_scene_destroy", "vx_scene_info", "vx_trace", "vx_trace_counters", "vx_render",
+            "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
+            "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban")
+
+
+def _check(status: int, where: str) -> None:
+    if status != VX_OK:
+        raise VixenError(status, where, (LIB.vx_last_error() or b"").decode())
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# --------------------------------------------------------------------------
+# Host-side scene build and generators
+# --------------------------------------------------------------------------
+@dataclass
+class BuiltScene:
+    """Scene buffers in VIXEN's layout (copied out of the C builder)."""
+    nodes: np.ndarray        # (node_count, 2) uint32
+    brick_data: np.ndarray   # (brick_count, 512) uint32
+    bc1: np.ndarray | None   # (brick_count, 32) uint64
+    brick_coords: np.ndarray  # (brick_count, 3) int32 grid cell per brick id
+    internal_count: int
+    config: VxOctreeConfig
+
+    @property
+    def N(self) -> int:
+        return int(self.config.bricks_per_axis) * 8
+
+    @property
+    def node_count(self) -> int:
+        return int(self.nodes.shape[0])
+
+    @property
+    def brick_count(self) -> int:
+        return int(self.brick_data.shape[0])
+
+    def device_bytes(self) -> int:
+        nb = self.brick_count
+        return self.node_count * 8 + nb * 2048 + nb * 64 + (nb * 256 if self.bc1 is not None else 0)
+
+
+def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0) -> BuiltScene:
+    """Dense RGBA8 grid grid[z, y, x] (0 = empty) -> octree buffers."""
+    grid = np.ascontiguousarray(grid, dtype=np.uint32)
+    N = grid.shape[0]
+    assert grid.shape == (N, N, N)
+    out = C.POINTER(VxBuiltScene)()
+    _check(LIB.vx_build_from_dense(_ptr(grid), N, int(bc1), nthreads, C.byref(out)), "vx_build_from_dense")
+    try:
+        s = out.contents
+        nc, nb = int(s.node_count), int(s.brick_count)
+        nodes = np.ctypeslib.as_array(s.esvo_nodes, shape=(nc * 2,)).reshape(nc, 2).copy()
+        bd = (np.ctypeslib.as_array(s.brick_data, shape=(nb * 512,)).reshape(nb, 512).copy() if nb
+              else np.zeros((0, 512), np.uint32))
+        b1 = None
+        if bc1:
+            b1 = (np.ctypeslib.as_array(s.bc1_colors, shape=(nb * 32,)).reshape(nb, 32).copy() if nb
+                  else np.zeros((0, 32), np.uint64))
+        packed = (np.ctypeslib.as_array(s.brick_coords, shape=(nb,)).copy() if nb else np.zeros(0, np.uint32))
+        coords = np.stack([packed & 1023, (packed >> 10) & 1023, (packed >> 20) & 1023], axis=1).astype(np.int32)
+        cfg = VxOctreeConfig()
+        C.memmove(C.addressof(cfg), C.addressof(s.config), C.sizeof(cfg))
+        return BuiltScene(nodes, bd, b1, coords, int(s.internal_count), cfg)
+    finally:
+        LIB.vx_built_scene_free(out)
+
+
+def bc1_encode_block(colors: np.ndarray) -> int:
+    c = np.ascontiguousarray(colors, dtype=np.float32).reshape(16, 3)
+    return int(LIB.vx_bc1_encode_block(_ptr(c)))
+
+
+def _gen(fn, where, N, *args):
+    grid = np.empty((N, N, N), dtype=np.uint32)
+    cam = VxCamera()
+    _check(fn(N, *args, _ptr(grid), C.byref(cam)), where)
+    return grid, cam
+
+
+def gen_cornell(N: int = 64, seed: int = 1) -> tuple[np.ndarray, VxCamera]:
+    return _gen(LIB.vx_gen_cornell, "vx_gen_cornell", N, seed)
+
+
+def gen_cave(N: int = 256, seed: int = 2, occupancy: float = 0.5) -> tuple[np.ndarray, VxCamera]:
+    return _gen(LIB.vx_gen_cave, "vx_gen_cave", N, seed, float(occupancy))
+
+
+def gen_urban(N: int = 1024, seed: int = 3, occupancy: float = 1.0) -> tuple[np.ndarray, VxCamera]:
+    return _gen(LIB.vx_gen_urban, "vx_gen_urban", N, seed, float(occupancy))
+
+
+# --------------------------------------------------------------------------
+# Device side
+# --------------------------------------------------------------------------
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("vbvs--vixen_amd: no HIP device visible")
+    return torch
+
+
+def _stream_ptr(stream) -> int | None:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream) or None
+
+
+class Context:
+    """One device (replaces the Vulkan device/queue around ComputeDispatchNode)."""
+
+    def __init__(self, device: int = 0):
+        _torch()
+        self.device = device
+        h = C.c_void_p()
+        _check(LIB.vx_ctx_create(device, C.byref(h)), "vx_ctx_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene uploaded to HBM (VoxelSceneCacher::UploadToGPU)."""
+
+    def __init__(self, ctx: Context, built: BuiltScene):
+        self.ctx = ctx
+        self.built = built
+        desc = VxSceneDesc()
+        desc.esvo_nodes = _ptr(built.nodes)
+        desc.node_count = built.node_count
+        desc.brick_data = _ptr(built.brick_data) if built.brick_count else None
+        desc.brick_count = built.brick_count
+        desc.bc1_colors = _ptr(built.bc1) if (built.bc1 is not None and built.brick_count) else None
+        self._cfg = built.config
+        desc.config = C.pointer(self._cfg)
+        h = C.c_void_p()
+        _check(LIB.vx_scene_create(ctx._h, C.byref(desc), C.byref(h)), "vx_scene_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # rays/hits are torch uint8 or float32 CUDA tensors holding n * 32 bytes
+    def trace(self, rays, hits=None, stream=None):
+        torch = _torch()
+        n = rays.numel() * rays.element_size() // 32
+        if hits is None:
+            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        _check(LIB.vx_trace(self._h, rays.data_ptr(), n, hits.data_ptr(), _stream_ptr(stream)), "vx_trace")
+        return hits
+
+    def trace_counters(self, rays, hits=None, stream=None):
+        torch = _torch()
+        n = rays.numel() * rays.element_size() // 32
+        if hits is None:
+            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        counters = torch.zeros(8, dtype=torch.int64, device=rays.device)
+        _check(LIB.vx_trace_counters(self._h, rays.data_ptr(), n, hits.data_ptr(), counters.data_ptr(),
+                                     _stream_ptr(stream)), "vx_trace_counters")
+        return hits, counters
+
+    def render(self, cam: VxCamera, w: int, h: int, tmax: float = 1e30, hits=True, rgba=False, stream=None):
+        torch = _torch()
+        dev = torch.device("cuda", self.ctx.device)
+        hb = torch.empty(w * h * 32, dtype=torch.uint8, device=dev) if hits else None
+        ib = torch.empty(w * h, dtype=torch.int32, device=dev) if rgba else None
+        _check(LIB.vx_render(self._h, C.byref(cam), w, h, float(tmax), hb.data_ptr() if hb is not None else None,
+                             ib.data_ptr() if ib is not None else None, _stream_ptr(stream)), "vx_render")
+        return hb, ib
+
+
+def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None):
+    torch = _torch()
+    rays = torch.empty(w * h * 32, dtype=torch.uint8, device=torch.device("cuda", device))
+    _check(LIB.vx_camera_rays(C.byref(cam), w, h, float(tmax), rays.data_ptr(), _stream_ptr(stream)),
+           "vx_camera_rays")
+    return rays
+
+
+def hits_to_numpy(hits) -> np.ndarray:
+    """Device hit buffer -> structured numpy array (HIT_DTYPE)."""
+    return hits.cpu().numpy().view(HIT_DTYPE)
+
+
+def rays_to_device(rays: np.ndarray, device: int = 0):
+    torch = _torch()
+    raw = np.ascontiguousarray(rays).view(np.uint8).reshape(-1)
+    return torch.from_numpy(raw.copy()).to(torch.device("cuda", device))
+
+
+@dataclass
+class RayHit:
+    """ISVOStructure::RayHit (libraries/SVO/include/ISVOStructure.h:101-112), voxel-path fields."""
+    hit: bool
+    tMin: float
+    hitPoint: tuple[float, float, float]
+    normal: tuple[float, float, float]
+    brick: int
+    voxel: int
+    leaf_node: int
+    color: tuple[float, float, float]
+
+
+class LaineKarrasOctreeHip:
+    """castRay surface of LaineKarrasOctree (libraries/SVO/src/SVOTraversal.cpp:189-198) on the GPU."""
+
+    def __init__(self, scene: Scene):
+        self.scene = scene
+
+    def castRays(self, rays: np.ndarray) -> np.ndarray:
+        d = rays_to_device(np.asarray(rays, dtype=RAY_DTYPE), self.scene.ctx.device)
+        out = hits_to_numpy(self.scene.trace(d))
+        _torch().cuda.synchronize()
+        return out
+
+    def castRay(self, origin, direction, tMin: float = 0.0, tMax: float = 1e30) -> RayHit:
+        r = np.zeros(1, RAY_DTYPE)
+        r["o"][0], r["d"][0], r["tmin"][0], r["tmax"][0] = origin, direction, tMin, tMax
+        h = self.castRays(r)[0]
+        hit = bool(h["flags"] & HIT_FLAG)
+        o = np.asarray(origin, np.float64)
+        dd = np.asarray(direction, np.float64)
+        dd = dd / (np.linalg.norm(dd) or 1.0)
+        normal = [0.0, 0.0, 0.0]
+        if hit:
+            ax = (int(h["flags"]) >> 1) & 3
+            normal[ax] = 1.0 if (int(h["flags"]) >> 3) & 1 else -1.0
+        t = float(h["t"])
+        hp = tuple(o + dd * t) if hit else (math.nan, math.nan, math.nan)
+        return RayHit(hit, t, hp, tuple(normal), int(h["brick"]), int(h["voxel"]), int(h["leaf_node"]),
+                      tuple(float(x) for x in h["rgb"]))
+
+
+def version() -> str:
+    return LIB.vx_version().decode()

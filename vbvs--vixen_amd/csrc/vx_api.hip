@@ -1,0 +1,477 @@
+// vx_api.hip -- the C ABI of libvixen_hip.so: contexts, scene upload and the
+// trace/render launches for gfx950.
+//
+// Launch geometry (all kernels): 256-lane workgroups (4 x wave64), one ray per
+// lane. Workgroup ids are remapped so that the 8 XCDs each own a contiguous
+// band of tiles/ray chunks (the hardware deals workgroup i to XCD i % 8), which
+// keeps neighbouring rays -- and the descriptors they share -- in one XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vixen_hip.h"
+#include "vx_common.hpp"
+#include "vx_trace.hpp"
+
+namespace vx {
+
+static thread_local std::string g_last_error;
+
+vx_status fail(vx_status code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+#define VX_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(VX_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_));      \
+    } while (0)
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kXcds = 8;
+
+struct CamParams {
+    float pos[3], dir[3], up[3], right[3];
+    float tan_half, aspect;
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Occupancy bit mask: word w of a brick holds voxels 32w..32w+31 (bit j = voxel 32w+j).
+__global__ void __launch_bounds__(kBlock) k_build_occ(const uint32_t* __restrict__ brick_data, uint64_t words,
+                                                      uint32_t* __restrict__ occ)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= words) return;
+    const uint4* src = reinterpret_cast<const uint4*>(brick_data + i * 32u);
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint4 v = src[q];
+        m |= (v.x != 0u ? 1u : 0u) << (4 * q + 0);
+        m |= (v.y != 0u ? 1u : 0u) << (4 * q + 1);
+        m |= (v.z != 0u ? 1u : 0u) << (4 * q + 2);
+        m |= (v.w != 0u ? 1u : 0u) << (4 * q + 3);
+    }
+    occ[i] = m;
+}
+
+__device__ __forceinline__ void store_hit(vx_hit* __restrict__ out, uint64_t i, const vx_hit& h)
+{
+    uint4* dst = reinterpret_cast<uint4*>(out + i);
+    dst[0] = make_uint4(h.flags, __float_as_uint(h.t), h.leaf_node, h.brick);
+    dst[1] = make_uint4(h.voxel, __float_as_uint(h.rgb[0]), __float_as_uint(h.rgb[1]), __float_as_uint(h.rgb[2]));
+}
+
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t per_xcd)
+{
+    return (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v)
+{
+    unsigned long long x = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63u) == 0u) atomicAdd(dst, x);
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __restrict__ rays, uint64_t n,
+                                                  uint32_t per_xcd, vx_hit* __restrict__ hits,
+                                                  unsigned long long* __restrict__ counters)
+{
+    const uint64_t i = (uint64_t)xcd_chunk(per_xcd) * kBlock + threadIdx.x;
+    TraceCounts cnt{0u, 0u, 0u, 0u};
+    if (i < n) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0], b = rp[1];
+        const float o[3] = {a.x, a.y, a.z}, d[3] = {b.x, b.y, b.z};
+        vx_hit h;
+        trace_ray<COUNT>(s, o, d, a.w, b.w, h, cnt);
+        store_hit(hits, i, h);
+    }
+    if (COUNT) {
+        wave_add_u64(&counters[0], i < n ? 1u : 0u);
+        wave_add_u64(&counters[1], cnt.iters);
+        wave_add_u64(&counters[2], cnt.leaves);
+        wave_add_u64(&counters[3], cnt.voxels);
+        wave_add_u64(&counters[4], cnt.hits);
+    }
+}
+
+// Pinhole ray of pixel (px, py): getRayDir (shaders/RayGeneration.glsl:61-71) at
+// the pixel centre (VoxelRayMarch_Compressed.comp:501-503). Unnormalised; the
+// trace normalises it.
+__device__ __forceinline__ void camera_dir(const CamParams& c, uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                                           float d[3], float& v)
+{
+    const float u = ((float)px + 0.5f) / (float)w;
+    v = ((float)py + 0.5f) / (float)h;
+    const float ndcx = u * 2.0f - 1.0f;
+    const float ndcy = -(v * 2.0f - 1.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a = ((c.right[i] * ndcx) * c.tan_half) * c.aspect;
+        const float b = (c.up[i] * ndcy) * c.tan_half;
+        d[i] = (c.dir[i] + a) + b;
+    }
+}
+
+__device__ __forceinline__ uint32_t to_u8(float x)
+{
+    x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    return (uint32_t)(x * 255.0f + 0.5f);
+}
+
+// 16x16 pixel tile per workgroup; each wave covers an 8x8 sub-tile.
+__global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uint32_t w, uint32_t h, float tmax,
+                                                   uint32_t tiles_x, uint32_t ntiles, uint32_t per_xcd,
+                                                   vx_hit* __restrict__ hits, uint32_t* __restrict__ rgba)
+{
+    const uint32_t tile = xcd_chunk(per_xcd);
+    if (tile >= ntiles) return;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    if (px >= w || py >= h) return;
+    float d[3], v;
+    camera_dir(c, px, py, w, h, d, v);
+    vx_hit hr;
+    TraceCounts cnt{0u, 0u, 0u, 0u};
+    const bool hit = trace_ray<false>(s, c.pos, d, 0.0f, tmax, hr, cnt);
+    const uint64_t i = (uint64_t)py * w + px;
+    if (hits) store_hit(hits, i, hr);
+    if (rgba) {
+        float col[3];
+        if (hit) {
+            // computeLighting (shaders/Lighting.glsl:19-28) with the entry-face normal
+            const uint32_t axis = (hr.flags >> 1) & 3u;
+            const float sgn = (hr.flags & 8u) ? 1.0f : -1.0f;
+            const float l = 0.57735026918962576f; // normalize(1,1,-1) components have magnitude 1/sqrt(3)
+            const float ldot = axis == 2u ? -sgn * l : sgn * l;
+            const float k = 0.3f + (ldot > 0.0f ? ldot : 0.0f) * 0.7f;
+            col[0] = hr.rgb[0] * k;
+            col[1] = hr.rgb[1] * k;
+            col[2] = hr.rgb[2] * k;
+        } else {
+            // sky (VoxelRayMarch_Compressed.comp:544)
+            const float g = 1.0f - v * 0.5f;
+            col[0] = 0.5f * g;
+            col[1] = 0.7f * g;
+            col[2] = 1.0f * g;
+        }
+        rgba[i] = to_u8(col[0]) | (to_u8(col[1]) << 8) | (to_u8(col[2]) << 16) | 0xFF000000u;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_camera_rays(CamParams c, uint32_t w, uint32_t h, float tmax,
+                                                        vx_ray* __restrict__ rays)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (uint64_t)w * h) return;
+    const uint32_t px = (uint32_t)(i % w), py = (uint32_t)(i / w);
+    float d[3], v;
+    camera_dir(c, px, py, w, h, d, v);
+    float4* dst = reinterpret_cast<float4*>(rays + i);
+    dst[0] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
+    dst[1] = make_float4(d[0], d[1], d[2], tmax);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static CamParams make_cam(const vx_camera* cam)
+{
+    CamParams c;
+    for (int i = 0; i < 3; ++i) {
+        c.pos[i] = cam->pos[i];
+        c.dir[i] = cam->dir[i];
+        c.up[i] = cam->up[i];
+        c.right[i] = cam->right[i];
+    }
+    // tan(radians(fov * 0.5)), radians(x) = x * pi/180 (RayGeneration.glsl:62)
+    c.tan_half = std::tan((cam->fov_deg * 0.5f) * 0.017453292519943295f);
+    c.aspect = cam->aspect;
+    return c;
+}
+
+} // namespace vx
+
+struct vx_ctx {
+    int device;
+};
+
+struct vx_scene {
+    vx_ctx* ctx;
+    uint2* nodes;
+    uint32_t* brick_data;
+    uint32_t* occ;
+    uint2* bc1;
+    uint64_t node_count;
+    uint64_t brick_count;
+    vx_octree_config cfg;
+    vx::SceneView view;
+};
+
+namespace vx {
+
+static int ilog2(uint32_t v)
+{
+    int l = 0;
+    while ((1u << l) < v) ++l;
+    return l;
+}
+
+// Walks the tree from the root and checks every pointer the traversal will
+// follow: internal levels carry no leaf bits, the brick-parent level carries
+// leafMask == validMask, child blocks and brick ids are in range.
+static vx_status validate_tree(const vx_scene_desc* d, int levels)
+{
+    const uint32_t* n = d->esvo_nodes;
+    const uint64_t nc = d->node_count;
+    struct Item {
+        uint64_t idx;
+        int depth;
+    };
+    std::vector<Item> todo;
+    todo.push_back({0, 0});
+    uint64_t visits = 0;
+    const uint64_t max_visits = nc * 8 + 16;
+    while (!todo.empty()) {
+        const Item it = todo.back();
+        todo.pop_back();
+        if (++visits > max_visits) return fail(VX_ERR_INVALID_SCENE, "tree has more paths than descriptors allow");
+        const uint32_t w0 = n[2 * it.idx], w1 = n[2 * it.idx + 1];
+        const uint32_t valid = (w0 >> 16) & 0xFFu, leaf = (w0 >> 24) & 0xFFu;
+        if (valid == 0) continue;
+        const uint64_t cp = (w0 & 0x8000u) ? (w1 & 0xFFFFFFu) : (w0 & 0x7FFFu);
+        const bool last = it.depth == levels - 1;
+        if (last ? (leaf != valid) : (leaf != 0))
+            return fail(VX_ERR_INVALID_SCENE, "leaf bits must mark exactly the children of brick-parent nodes (node " +
+                                                  std::to_string(it.idx) + ")");
+        const uint32_t cnt = static_cast<uint32_t>(__builtin_popcount(valid));
+        if (cp + cnt > nc)
+            return fail(VX_ERR_INVALID_SCENE, "child block of node " + std::to_string(it.idx) + " out of range");
+        for (uint32_t c = 0; c < cnt; ++c) {
+            if (last) {
+                const uint32_t brick = n[2 * (cp + c) + 1] & 0xFFFFFFu;
+                if (brick != 0xFFFFFFu && brick >= d->brick_count)
+                    return fail(VX_ERR_INVALID_SCENE, "brick id out of range at descriptor " + std::to_string(cp + c));
+            } else {
+                todo.push_back({cp + c, it.depth + 1});
+            }
+        }
+    }
+    return VX_OK;
+}
+
+static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out)
+{
+    if (!ctx || !d || !out || !d->config || !d->esvo_nodes) return fail(VX_ERR_INVALID_ARG, "vx_scene_create: null argument");
+    *out = nullptr;
+    const vx_octree_config& c = *d->config;
+    const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
+    if (c.bricks_per_axis < 2 || c.bricks_per_axis > 512 || (B & (B - 1)) != 0)
+        return fail(VX_ERR_INVALID_SCENE, "bricks_per_axis must be a power of two in [2, 512]");
+    const int levels = ilog2(B);
+    if (c.brick_size != 8 || c.brick_depth_levels != 3 || c.user_max_levels != levels + 3 || c.esvo_max_scale != 22)
+        return fail(VX_ERR_INVALID_SCENE, "config must describe 8^3 bricks under log2(bricks_per_axis) levels, esvo scale 22");
+    if (levels - 1 > VX_STACK) return fail(VX_ERR_UNSUPPORTED, "tree deeper than the register stack");
+    const float sx = c.grid_max[0] - c.grid_min[0], sy = c.grid_max[1] - c.grid_min[1], sz = c.grid_max[2] - c.grid_min[2];
+    if (!(sx > 0.0f) || sx != sy || sx != sz || !std::isfinite(sx))
+        return fail(VX_ERR_INVALID_SCENE, "grid bounds must be a non-empty cube");
+    if (d->node_count < 1 || d->node_count > 0xFFFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "node_count out of range");
+    if (d->brick_count >= 0xFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "brick_count exceeds 24-bit ids");
+    if (d->brick_count > 0 && !d->brick_data) return fail(VX_ERR_INVALID_ARG, "brick_data is null");
+    if (vx_status st = validate_tree(d, levels); st != VX_OK) return st;
+
+    int cur = 0;
+    VX_HIP(hipGetDevice(&cur));
+    if (cur != ctx->device) VX_HIP(hipSetDevice(ctx->device));
+
+    auto* s = new (std::nothrow) vx_scene{};
+    if (!s) return fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
+    s->ctx = ctx;
+    s->node_count = d->node_count;
+    s->brick_count = d->brick_count;
+    s->cfg = c;
+    auto cleanup = [&](vx_status st) {
+        (void)hipFree(s->nodes);
+        (void)hipFree(s->brick_data);
+        (void)hipFree(s->occ);
+        (void)hipFree(s->bc1);
+        delete s;
+        return st;
+    };
+    const uint64_t nb = d->brick_count;
+    const size_t node_bytes = d->node_count * 8, brick_bytes = std::max<uint64_t>(1, nb) * 2048,
+                 occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256;
+    if (hipMalloc(&s->nodes, node_bytes) != hipSuccess || hipMalloc(&s->brick_data, brick_bytes) != hipSuccess ||
+        hipMalloc(&s->occ, occ_bytes) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess))
+        return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
+    if (hipMemcpy(s->nodes, d->esvo_nodes, node_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        (nb && hipMemcpy(s->brick_data, d->brick_data, nb * 2048, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nb && d->bc1_colors && hipMemcpy(s->bc1, d->bc1_colors, nb * 256, hipMemcpyHostToDevice) != hipSuccess))
+        return cleanup(fail(VX_ERR_HIP, "vx_scene_create: upload failed"));
+    if (nb) {
+        const uint64_t words = nb * 16;
+        k_build_occ<<<dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0>>>(s->brick_data, words, s->occ);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            return cleanup(fail(VX_ERR_HIP, "vx_scene_create: occupancy mask build failed"));
+    }
+    SceneView& v = s->view;
+    v.nodes = s->nodes;
+    v.brick_data = s->brick_data;
+    v.occ = s->occ;
+    v.bc1 = d->bc1_colors ? s->bc1 : nullptr;
+    v.node_count = static_cast<uint32_t>(d->node_count);
+    v.brick_count = static_cast<uint32_t>(nb);
+    const uint32_t N = B * 8;
+    for (int i = 0; i < 3; ++i) v.gmin[i] = c.grid_min[i];
+    v.Nf = static_cast<float>(N);
+    v.k = v.Nf / sx;
+    v.invN = 1.0f / v.Nf;
+    v.Bf = static_cast<float>(B);
+    v.min_scale = 22 - c.user_max_levels + 1;
+    *out = s;
+    return VX_OK;
+}
+
+static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
+                              hipStream_t st)
+{
+    if (!s || (n && (!rays || !hits))) return fail(VX_ERR_INVALID_ARG, "vx_trace: null argument");
+    if (n == 0) return VX_OK;
+    const uint64_t chunks = (n + kBlock - 1) / kBlock;
+    const uint64_t per = (chunks + kXcds - 1) / kXcds;
+    if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
+    const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
+    if (counters) {
+        VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
+        k_trace<true><<<grid, block, 0, st>>>(s->view, rays, n, static_cast<uint32_t>(per), hits,
+                                              reinterpret_cast<unsigned long long*>(counters));
+    } else {
+        k_trace<false><<<grid, block, 0, st>>>(s->view, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+    }
+    VX_HIP(hipGetLastError());
+    return VX_OK;
+}
+
+} // namespace vx
+
+extern "C" {
+
+const char* vx_last_error(void) { return vx::g_last_error.c_str(); }
+
+const char* vx_version(void) { return "vixen-hip 0.1 gfx950 wave64 strict-fp32"; }
+
+vx_status vx_ctx_create(int device, vx_ctx** out)
+{
+    if (!out) return vx::fail(VX_ERR_INVALID_ARG, "vx_ctx_create: null out");
+    *out = nullptr;
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess)
+        return vx::fail(VX_ERR_HIP, std::string("vx_ctx_create: hipGetDeviceCount failed: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_ctx_create: device " + std::to_string(device) + " not in [0, " +
+                                                std::to_string(count) + ")");
+    if (hipSetDevice(device) != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_ctx_create: hipSetDevice failed");
+    auto* c = new (std::nothrow) vx_ctx{device};
+    if (!c) return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_ctx_create: out of memory");
+    *out = c;
+    return VX_OK;
+}
+
+vx_status vx_ctx_destroy(vx_ctx* ctx)
+{
+    delete ctx;
+    return VX_OK;
+}
+
+vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out)
+{
+    try {
+        return vx::scene_create(ctx, desc, out);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
+    } catch (...) {
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_create: unexpected exception");
+    }
+}
+
+vx_status vx_scene_destroy(vx_scene* s)
+{
+    if (!s) return VX_OK;
+    (void)hipFree(s->nodes);
+    (void)hipFree(s->brick_data);
+    (void)hipFree(s->occ);
+    (void)hipFree(s->bc1);
+    delete s;
+    return VX_OK;
+}
+
+vx_status vx_scene_info(const vx_scene* s, uint64_t* node_count, uint64_t* brick_count, int32_t* user_max_levels,
+                        int32_t* bricks_per_axis)
+{
+    if (!s) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_info: null scene");
+    if (node_count) *node_count = s->node_count;
+    if (brick_count) *brick_count = s->brick_count;
+    if (user_max_levels) *user_max_levels = s->cfg.user_max_levels;
+    if (bricks_per_axis) *bricks_per_axis = s->cfg.bricks_per_axis;
+    return VX_OK;
+}
+
+vx_status vx_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, vx_stream stream)
+{
+    return vx::launch_trace(s, rays, n, hits, nullptr, static_cast<hipStream_t>(stream));
+}
+
+vx_status vx_trace_counters(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
+                            vx_stream stream)
+{
+    if (!counters) return vx::fail(VX_ERR_INVALID_ARG, "vx_trace_counters: null counters");
+    return vx::launch_trace(s, rays, n, hits, counters, static_cast<hipStream_t>(stream));
+}
+
+vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, float tmax, vx_hit* hits,
+                    uint32_t* rgba8, vx_stream stream)
+{
+    if (!s || !cam || (!hits && !rgba8)) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: null argument");
+    if (w == 0 || h == 0) return VX_OK;
+    if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: image too large");
+    const vx::CamParams c = vx::make_cam(cam);
+    const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;
+    const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    vx::k_render<<<dim3(per * vx::kXcds), dim3(vx::kBlock), 0, st>>>(s->view, c, w, h, tmax, tiles_x, ntiles, per,
+                                                                     hits, rgba8);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
+    return VX_OK;
+}
+
+vx_status vx_camera_rays(const vx_camera* cam, uint32_t w, uint32_t h, float tmax, vx_ray* rays, vx_stream stream)
+{
+    if (!cam || !rays) return vx::fail(VX_ERR_INVALID_ARG, "vx_camera_rays: null argument");
+    const uint64_t n = (uint64_t)w * h;
+    if (n == 0) return VX_OK;
+    const vx::CamParams c = vx::make_cam(cam);
+    vx::k_camera_rays<<<dim3((unsigned)((n + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0,
+                        static_cast<hipStream_t>(stream)>>>(c, w, h, tmax, rays);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_camera_rays: launch failed");
+    return VX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,478 @@
+// vx_trace.hpp -- device-side traversal for gfx950: ESVO -> brick DDA -> colour.
+//
+// One ray per lane of a wave64. The arithmetic is strict FP32 (the library is
+// compiled with -ffp-contract=off and correctly rounded divide/sqrt) and keeps
+// the evaluation order of the reference traversal so results are bit-identical
+// to a strict-FP32 CPU evaluation of the same steps:
+//   ray set-up / volume entry : libraries/SVO/src/SVOTraversal.cpp:117-149,238-259,516-583
+//   coefficients              : libraries/SVO/src/SVOTraversal.cpp:699-728
+//   init/PUSH/ADVANCE/POP     : shaders/ESVOTraversal.glsl:84-372
+//   loop + t_max restore      : shaders/VoxelRayMarch_Compressed.comp:396-478
+//   leaf descriptor           : shaders/VoxelRayMarch_Compressed.comp:291-305
+//   brick cell from ESVO pos  : libraries/SVO/src/SVOBrickDDA.cpp:85-93
+//   brick DDA, t, normal      : libraries/SVO/src/SVOBrickDDA.cpp:187-225,390-533
+//   BC1 decode                : libraries/VoxelData/src/Compression/DXT1Compressor.cpp:14-36
+//
+// Layout choices that are this backend's own:
+//   - the traversal stack only holds the scales a well-formed tree can reach
+//     (22 down to one above the brick level, at most VX_STACK entries) and lives
+//     in VGPRs: every access is a fully unrolled compare/select, never scratch;
+//   - brick occupancy is read from a 64-byte-per-brick bit mask derived from
+//     brickData at upload (bit lin of the brick's 16 words), so a brick's DDA
+//     touches one cache line instead of 2 KiB; the hit voxel's brickData word
+//     is read only when the scene has no BC1 colours.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vixen_hip.h"
+
+namespace vx {
+
+constexpr int VX_STACK = 8;          // supports up to 9 levels above the bricks (N <= 4096)
+constexpr int ESVO_MAX_SCALE = 22;
+constexpr int MAX_ITERS = 512;       // shaders/ESVOTraversal.glsl:25
+constexpr float T_EPS = 1e-6f;       // shaders/ESVOTraversal.glsl:26
+constexpr float DIR_EPS = 1e-5f;     // shaders/ESVOTraversal.glsl:27, SVOTraversal.cpp:707
+
+struct SceneView {
+    const uint2* __restrict__ nodes;
+    const uint32_t* __restrict__ brick_data;  // 512 words per brick (colour when no BC1)
+    const uint32_t* __restrict__ occ;         // 16 words per brick: occupancy bits
+    const uint2* __restrict__ bc1;            // 32 blocks per brick or nullptr
+    uint32_t node_count;
+    uint32_t brick_count;
+    float gmin[3];
+    float k;          // N / (gmax.x - gmin.x): world -> voxel units
+    float Nf;         // N
+    float invN;       // 1 / N (a power of two)
+    float Bf;         // bricks per axis
+    int min_scale;
+};
+
+struct TraceCounts {
+    uint32_t iters, leaves, voxels, hits;
+};
+
+__device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
+
+__device__ __forceinline__ void store_miss(vx_hit& h, uint32_t iters)
+{
+    h.flags = iters << 16;
+    h.t = __builtin_inff();
+    h.leaf_node = 0xFFFFFFFFu;
+    h.brick = 0xFFFFFFFFu;
+    h.voxel = 0xFFFFFFFFu;
+    h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
+}
+
+// BC1 decode of one texel (integer extraction, u32->f32 round-to-nearest, x 2^-32,
+// blend with fl(2/3), fl(1/3), no contraction).
+__device__ __forceinline__ void bc1_decode(uint2 blk, uint32_t texel, float rgb[3])
+{
+    const uint32_t head = blk.x, bits = blk.y;
+    const float S = 0x1p-32f;
+    const uint32_t sel = (bits >> (2u * texel)) & 3u;
+    const float r0 = (float)(head << 16) * S, g0 = (float)(head << 21) * S, b0 = (float)(head << 27) * S;
+    const float r1 = (float)head * S, g1 = (float)(head << 5) * S, b1 = (float)(head << 11) * S;
+    const float c23 = 2.0f / 3.0f, c13 = 1.0f / 3.0f;
+    if (sel == 0u) {
+        rgb[0] = r0; rgb[1] = g0; rgb[2] = b0;
+    } else if (sel == 1u) {
+        rgb[0] = r1; rgb[1] = g1; rgb[2] = b1;
+    } else if (sel == 2u) {
+        rgb[0] = r0 * c23 + r1 * c13;
+        rgb[1] = g0 * c23 + g1 * c13;
+        rgb[2] = b0 * c23 + b1 * c13;
+    } else {
+        rgb[0] = r0 * c13 + r1 * c23;
+        rgb[1] = g0 * c13 + g1 * c23;
+        rgb[2] = b0 * c13 + b1 * c23;
+    }
+}
+
+__device__ __forceinline__ void rgba8_decode(uint32_t w, float rgb[3])
+{
+    rgb[0] = (float)(w & 255u) / 255.0f;
+    rgb[1] = (float)((w >> 8) & 255u) / 255.0f;
+    rgb[2] = (float)((w >> 16) & 255u) / 255.0f;
+}
+
+// Ray set-up in the grid's local voxel space [0,N]^3. Returns false on a miss
+// or an invalid ray.
+__device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
+                                          float tmax, float ol[3], float dl[3], float du[3], float& tstart,
+                                          float& texit)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (!__builtin_isfinite(o[i]) || !__builtin_isfinite(d[i])) return false;
+    const float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (!(len2 > 0.0f) || !__builtin_isfinite(len2)) return false;
+    const float inv = 1.0f / __builtin_sqrtf(len2);
+    float invd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        du[i] = d[i] * inv;
+        if (!__builtin_isfinite(du[i])) return false;
+        ol[i] = (o[i] - s.gmin[i]) * s.k;
+        dl[i] = du[i] * s.k;
+        if (__builtin_fabsf(dl[i]) < 1e-8f) {
+            if (ol[i] < 0.0f || ol[i] > s.Nf) return false;
+            invd[i] = (dl[i] >= 0.0f) ? 1e20f : -1e20f;
+        } else {
+            invd[i] = 1.0f / dl[i];
+        }
+    }
+    float tn[3], tf[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float t0 = (0.0f - ol[i]) * invd[i];
+        const float t1 = (s.Nf - ol[i]) * invd[i];
+        tn[i] = mn(t0, t1);
+        tf[i] = mx(t0, t1);
+    }
+    float tEntry = mx(mx(tn[0], tn[1]), tn[2]);
+    float tExit = mn(mn(tf[0], tf[1]), tf[2]);
+    if (!(tEntry <= tExit && tExit >= 0.0f)) return false;
+    tEntry = mx(tEntry, tmin);
+    tExit = mn(tExit, tmax);
+    if (tEntry >= tExit || tExit < 0.0f) return false;
+    tstart = mx(tEntry, 0.0f);
+    texit = tExit;
+    return true;
+}
+
+__device__ __forceinline__ float corrected_tc_max(float tx, float ty, float tz, const float du[3], float t_max)
+{
+    const float thr = 1000.0f;
+    const float vx = (__builtin_fabsf(du[0]) >= DIR_EPS && __builtin_fabsf(tx) < thr) ? tx : t_max;
+    const float vy = (__builtin_fabsf(du[1]) >= DIR_EPS && __builtin_fabsf(ty) < thr) ? ty : t_max;
+    const float vz = (__builtin_fabsf(du[2]) >= DIR_EPS && __builtin_fabsf(tz) < thr) ? tz : t_max;
+    return mn(mn(vx, vy), vz);
+}
+
+// Brick DDA in local voxel space. Returns true on a hit and fills the hit record.
+template <bool COUNT>
+__device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, const int bc[3], const float ol[3],
+                                          const float dl[3], float tstart, float texit, vx_hit& h,
+                                          TraceCounts& cnt)
+{
+    float bmin[3], tn[3], tf[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        bmin[i] = (float)(bc[i] * 8);
+        const float invd = (__builtin_fabsf(dl[i]) < 1e-8f) ? ((dl[i] >= 0.0f) ? 1e8f : -1e8f) : 1.0f / dl[i];
+        const float bmax = bmin[i] + 8.0f;
+        const float t0 = (bmin[i] - ol[i]) * invd;
+        const float t1 = (bmax - ol[i]) * invd;
+        tn[i] = mn(t0, t1);
+        tf[i] = mx(t0, t1);
+    }
+    float btmin = mx(mx(tn[0], tn[1]), tn[2]);
+    float btmax = mn(mn(tf[0], tf[1]), tf[2]);
+    btmin = mx(btmin, tstart);
+    btmax = mn(btmax, texit);
+    if (btmin > btmax) return false;
+
+    int cur[3], step[3];
+    float tDelta[3], tNext[3], entry[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        entry[i] = ol[i] + dl[i] * btmin;
+        const int c = (int)__builtin_floorf(entry[i] - bmin[i]);
+        cur[i] = c < 0 ? 0 : (c > 7 ? 7 : c);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (__builtin_fabsf(dl[i]) < 1e-8f) {
+            step[i] = 0;
+            tDelta[i] = 3.402823466e+38f;
+            tNext[i] = 3.402823466e+38f;
+        } else if (dl[i] > 0.0f) {
+            step[i] = 1;
+            tDelta[i] = 1.0f / __builtin_fabsf(dl[i]);
+            const float nb = bmin[i] + (float)(cur[i] + 1);
+            tNext[i] = btmin + (nb - entry[i]) / dl[i];
+        } else {
+            step[i] = -1;
+            tDelta[i] = 1.0f / __builtin_fabsf(dl[i]);
+            const float nb = bmin[i] + (float)cur[i];
+            tNext[i] = btmin + (entry[i] - nb) / __builtin_fabsf(dl[i]);
+        }
+    }
+    const uint32_t* occ = s.occ + (size_t)brick * 16u;
+    for (int it = 0; it < 24; ++it) {
+        if ((uint32_t)cur[0] > 7u || (uint32_t)cur[1] > 7u || (uint32_t)cur[2] > 7u) return false;
+        const uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
+        if (COUNT) cnt.voxels += 1;
+        const uint32_t word = occ[lin >> 5];
+        if ((word >> (lin & 31u)) & 1u) {
+            float tnh[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (__builtin_fabsf(dl[i]) < 1e-8f) {
+                    tnh[i] = -__builtin_inff();
+                } else {
+                    const float vmin = bmin[i] + (float)cur[i];
+                    const float vmax = vmin + 1.0f;
+                    const float t0 = (vmin - ol[i]) / dl[i];
+                    const float t1 = (vmax - ol[i]) / dl[i];
+                    tnh[i] = mn(t0, t1);
+                }
+            }
+            float tHit = mx(mx(tnh[0], tnh[1]), tnh[2]);
+            tHit = mx(tHit, tstart);
+            uint32_t axis;
+            if (tnh[0] >= tnh[1] && tnh[0] >= tnh[2]) axis = 0u;
+            else if (tnh[1] >= tnh[2]) axis = 1u;
+            else axis = 2u;
+            const float da = axis == 0u ? dl[0] : (axis == 1u ? dl[1] : dl[2]);
+            const uint32_t signpos = (da > 0.0f) ? 0u : 1u;
+            h.flags = 1u | (axis << 1) | (signpos << 3);
+            h.t = tHit;
+            h.brick = brick;
+            h.voxel = lin;
+            if (s.bc1) bc1_decode(s.bc1[(size_t)brick * 32u + (lin >> 4)], lin & 15u, h.rgb);
+            else rgba8_decode(s.brick_data[(size_t)brick * 512u + lin], h.rgb);
+            return true;
+        }
+        if (tNext[0] < tNext[1] && tNext[0] < tNext[2]) {
+            if (tNext[0] > btmax) return false;
+            cur[0] += step[0];
+            tNext[0] += tDelta[0];
+        } else if (tNext[1] < tNext[2]) {
+            if (tNext[1] > btmax) return false;
+            cur[1] += step[1];
+            tNext[1] += tDelta[1];
+        } else {
+            if (tNext[2] > btmax) return false;
+            cur[2] += step[2];
+            tNext[2] += tDelta[2];
+        }
+    }
+    return false;
+}
+
+// VGPR-resident stack indexed by k = 22 - scale; every access is unrolled so
+// the arrays never spill to scratch.
+struct RegStack {
+    uint32_t ptr[VX_STACK];
+    float tmax[VX_STACK];
+    __device__ __forceinline__ void fill(float t)
+    {
+#pragma unroll
+        for (int i = 0; i < VX_STACK; ++i) {
+            ptr[i] = 0u;
+            tmax[i] = t;
+        }
+    }
+    __device__ __forceinline__ void put(int k, uint32_t p, float t)
+    {
+#pragma unroll
+        for (int i = 0; i < VX_STACK; ++i)
+            if (i == k) {
+                ptr[i] = p;
+                tmax[i] = t;
+            }
+    }
+    __device__ __forceinline__ float get_tmax(int k) const
+    {
+        float r = tmax[0];
+#pragma unroll
+        for (int i = 1; i < VX_STACK; ++i)
+            if (i == k) r = tmax[i];
+        return r;
+    }
+    __device__ __forceinline__ uint32_t get_ptr(int k) const
+    {
+        uint32_t r = ptr[0];
+#pragma unroll
+        for (int i = 1; i < VX_STACK; ++i)
+            if (i == k) r = ptr[i];
+        return r;
+    }
+};
+
+// Full traversal of one ray. Writes the hit record; returns true on a hit.
+template <bool COUNT>
+__device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
+                                          float tmax, vx_hit& h, TraceCounts& cnt)
+{
+    store_miss(h, 0u);
+    float ol[3], dl[3], du[3], tstart, texit;
+    if (!setup_ray(s, o, d, tmin, tmax, ol, dl, du, tstart, texit)) return false;
+
+    float n0[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) n0[i] = (ol[i] + dl[i] * tstart) * s.invN + 1.0f;
+
+    float tc[3], tb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float ds = du[i];
+        if (__builtin_fabsf(ds) < DIR_EPS) ds = __builtin_copysignf(DIR_EPS, ds);
+        tc[i] = 1.0f / -__builtin_fabsf(ds);
+        tb[i] = tc[i] * n0[i];
+    }
+    int mask = 7;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (du[i] > 0.0f) {
+            mask ^= 1 << i;
+            tb[i] = 3.0f * tc[i] - tb[i];
+        }
+
+    float t_min = mx(mx(2.0f * tc[0] - tb[0], 2.0f * tc[1] - tb[1]), 2.0f * tc[2] - tb[2]);
+    float t_max = mn(mn(tc[0] - tb[0], tc[1] - tb[1]), tc[2] - tb[2]);
+    t_min = mx(t_min, 0.0f);
+    if (t_min >= t_max) return false;
+
+    RegStack stk;
+    stk.fill(t_max);
+    uint32_t parent = 0u;
+    int scale = ESVO_MAX_SCALE;
+    float scale_exp2 = 0.5f;
+    float pos[3] = {1.0f, 1.0f, 1.0f};
+    int idx = 0;
+    const bool use_pos = t_min < 1e-4f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float m = (mask & (1 << i)) ? n0[i] : (3.0f - n0[i]);
+        if (__builtin_fabsf(du[i]) < DIR_EPS || use_pos) {
+            if (m >= 1.5f) {
+                idx |= 1 << i;
+                pos[i] = 1.5f;
+            }
+        } else if (1.5f * tc[i] - tb[i] > t_min) {
+            idx ^= 1 << i;
+            pos[i] = 1.5f;
+        }
+    }
+    const bool canx = __builtin_fabsf(du[0]) >= DIR_EPS, cany = __builtin_fabsf(du[1]) >= DIR_EPS,
+               canz = __builtin_fabsf(du[2]) >= DIR_EPS;
+
+    int iter;
+    for (iter = 0; iter < MAX_ITERS && scale <= ESVO_MAX_SCALE; ++iter) {
+        if (COUNT) cnt.iters += 1;
+        if (parent >= s.node_count) {
+            store_miss(h, (uint32_t)(iter + 1));
+            return false;
+        }
+        const uint2 desc = s.nodes[parent];
+        const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
+        const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
+        const int local = idx ^ ((~mask) & 7);
+
+        bool process = false;
+        float tv_max = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        if (((valid >> local) & 1u) && !(t_min > t_max + T_EPS)) {
+            const float tx = pos[0] * tc[0] - tb[0];
+            const float ty = pos[1] * tc[1] - tb[1];
+            const float tz = pos[2] * tc[2] - tb[2];
+            const float tcm = corrected_tc_max(tx, ty, tz, du, t_max);
+            tv_max = mn(t_max, tcm);
+            const float half = scale_exp2 * 0.5f;
+            cx = half * tc[0] + tx;
+            cy = half * tc[1] + ty;
+            cz = half * tc[2] + tz;
+            process = t_min <= tv_max + T_EPS;
+        }
+        if (process) {
+            if ((leaf >> local) & 1u) {
+                if (COUNT) cnt.leaves += 1;
+                const uint32_t below = (1u << local) - 1u;
+                const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
+                if (lidx < s.node_count) {
+                    const uint32_t brick = s.nodes[lidx].y & 0xFFFFFFu;
+                    if (brick != 0xFFFFFFu && brick < s.brick_count) {
+                        int bc[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const float lp = (mask & (1 << i)) ? pos[i] : (3.0f - scale_exp2 - pos[i]);
+                            bc[i] = (int)((lp - 1.0f) * s.Bf);
+                        }
+                        if (brick_dda<COUNT>(s, brick, bc, ol, dl, tstart, texit, h, cnt)) {
+                            h.flags |= (uint32_t)(iter + 1) << 16;
+                            h.leaf_node = lidx;
+                            if (COUNT) cnt.hits += 1;
+                            return true;
+                        }
+                    }
+                }
+                t_min = tv_max;
+            } else {
+                stk.put(ESVO_MAX_SCALE - scale, parent, t_max);
+                const uint32_t below = (1u << local) - 1u;
+                parent = cp + popc8(valid & ~leaf & below);
+                idx = 0;
+                scale -= 1;
+                const float half = scale_exp2 * 0.5f;
+                scale_exp2 = half;
+                if (cx > t_min) { idx ^= 1; pos[0] += scale_exp2; }
+                if (cy > t_min) { idx ^= 2; pos[1] += scale_exp2; }
+                if (cz > t_min) { idx ^= 4; pos[2] += scale_exp2; }
+                t_max = tv_max;
+                continue;
+            }
+        }
+
+        // ADVANCE
+        const float tx = pos[0] * tc[0] - tb[0];
+        const float ty = pos[1] * tc[1] - tb[1];
+        const float tz = pos[2] * tc[2] - tb[2];
+        float tcm = corrected_tc_max(tx, ty, tz, du, t_max);
+        if (tcm >= 1e10f) tcm = mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f);
+        int step_mask = 0;
+        if (canx && tx <= tcm) { step_mask ^= 1; pos[0] -= scale_exp2; }
+        if (cany && ty <= tcm) { step_mask ^= 2; pos[1] -= scale_exp2; }
+        if (canz && tz <= tcm) { step_mask ^= 4; pos[2] -= scale_exp2; }
+        t_min = mx(tcm, 0.0f);
+        idx ^= step_mask;
+
+        if ((idx & step_mask) == 0) {
+            if (scale < ESVO_MAX_SCALE) t_max = stk.get_tmax(ESVO_MAX_SCALE - (scale + 1));
+            continue;
+        }
+
+        // POP
+        if (scale >= ESVO_MAX_SCALE) {
+            if (t_min > t_max || pos[0] < 1.0f || pos[0] >= 2.0f || pos[1] < 1.0f || pos[1] >= 2.0f ||
+                pos[2] < 1.0f || pos[2] >= 2.0f) {
+                store_miss(h, (uint32_t)(iter + 1));
+                return false;
+            }
+            continue;
+        }
+        uint32_t diff = 0u;
+        if (step_mask & 1) diff |= f2u(pos[0]) ^ f2u(pos[0] + scale_exp2);
+        if (step_mask & 2) diff |= f2u(pos[1]) ^ f2u(pos[1] + scale_exp2);
+        if (step_mask & 4) diff |= f2u(pos[2]) ^ f2u(pos[2] + scale_exp2);
+        if (diff == 0u) {
+            store_miss(h, (uint32_t)(iter + 1));
+            return false;
+        }
+        scale = (int)((f2u((float)diff) >> 23) - 127u);
+        scale_exp2 = u2f((uint32_t)(scale - ESVO_MAX_SCALE - 1 + 127) << 23);
+        if (scale < s.min_scale || scale > ESVO_MAX_SCALE) {
+            store_miss(h, (uint32_t)(iter + 1));
+            return false;
+        }
+        parent = stk.get_ptr(ESVO_MAX_SCALE - scale);
+        t_max = stk.get_tmax(ESVO_MAX_SCALE - scale);
+        const uint32_t shx = f2u(pos[0]) >> scale, shy = f2u(pos[1]) >> scale, shz = f2u(pos[2]) >> scale;
+        pos[0] = u2f(shx << scale);
+        pos[1] = u2f(shy << scale);
+        pos[2] = u2f(shz << scale);
+        idx = (int)((shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2));
+    }
+    store_miss(h, (uint32_t)iter);
+    return false;
+}
+
+} // namespace vx
