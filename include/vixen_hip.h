@@ -154,6 +154,9 @@ vx_status vx_ctx_destroy(vx_ctx* ctx);
  * (bit 15) is set, its child block index is word1 bits 0-23 instead of the
  * 15-bit childPointer (the contour field is unused on the voxel path). */
 vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out);
+
+/* The checks vx_scene_create runs before uploading, without touching a device. */
+vx_status vx_scene_validate(const vx_scene_desc* desc);
 vx_status vx_scene_destroy(vx_scene* scene);
 
 /* Device addresses of a scene's pools (for callers that share them). */

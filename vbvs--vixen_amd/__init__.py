@@ -112,6 +112,7 @@ def _load() -> C.CDLL:
         "vx_ctx_create": ([C.c_int, C.POINTER(vp)], st),
         "vx_ctx_destroy": ([vp], st),
         "vx_scene_create": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
+        "vx_scene_validate": ([C.POINTER(VxSceneDesc)], st),
         "vx_scene_destroy": ([vp], st),
         "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
         "vx_trace": ([vp, vp, u64, vp, vp], st),
@@ -135,7 +136,7 @@ def _load() -> C.CDLL:
 LIB = _load()
 
 # Every symbol the headers declare (the CPU suite checks the library exports them all).
-EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create",
+EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create", "vx_scene_validate",
             "vx_scene_destroy", "vx_scene_info", "vx_trace", "vx_trace_counters", "vx_render",
             "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban")

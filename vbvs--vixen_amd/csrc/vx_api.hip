@@ -277,10 +277,9 @@ static vx_status validate_tree(const vx_scene_desc* d, int levels)
     return VX_OK;
 }
 
-static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out)
+static vx_status scene_validate(const vx_scene_desc* d)
 {
-    if (!ctx || !d || !out || !d->config || !d->esvo_nodes) return fail(VX_ERR_INVALID_ARG, "vx_scene_create: null argument");
-    *out = nullptr;
+    if (!d || !d->config || !d->esvo_nodes) return fail(VX_ERR_INVALID_ARG, "vx_scene_validate: null argument");
     const vx_octree_config& c = *d->config;
     const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
     if (c.bricks_per_axis < 2 || c.bricks_per_axis > 512 || (B & (B - 1)) != 0)
@@ -295,7 +294,17 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     if (d->node_count < 1 || d->node_count > 0xFFFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "node_count out of range");
     if (d->brick_count >= 0xFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "brick_count exceeds 24-bit ids");
     if (d->brick_count > 0 && !d->brick_data) return fail(VX_ERR_INVALID_ARG, "brick_data is null");
-    if (vx_status st = validate_tree(d, levels); st != VX_OK) return st;
+    return validate_tree(d, levels);
+}
+
+static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out)
+{
+    if (!ctx || !d || !out) return fail(VX_ERR_INVALID_ARG, "vx_scene_create: null argument");
+    *out = nullptr;
+    if (vx_status st = scene_validate(d); st != VX_OK) return st;
+    const vx_octree_config& c = *d->config;
+    const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
+    const float sx = c.grid_max[0] - c.grid_min[0];
 
     int cur = 0;
     VX_HIP(hipGetDevice(&cur));
@@ -409,6 +418,15 @@ vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out
         return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
     } catch (...) {
         return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_create: unexpected exception");
+    }
+}
+
+vx_status vx_scene_validate(const vx_scene_desc* desc)
+{
+    try {
+        return vx::scene_validate(desc);
+    } catch (...) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_validate: unexpected exception");
     }
 }
 
