@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); rays shard by image tile "
@@ -116,6 +118,13 @@ def main() -> int:
     rays = W * H
     mrays = rays / (ms_step * 1e-3) / 1e6
     log(f"[bench] {args.steps} launches: {ms_step:.4f} ms/launch, {mrays:.1f} Mrays/s (wall {wall:.3f}s)")
+
+    if args.calib_copy_mib:
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
 
     # algorithmic bytes from the kernel's counting variant on the same rays
     cam_rays = vx.camera_rays(cam, W, H, 1e30)
