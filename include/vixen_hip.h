@@ -159,7 +159,7 @@ vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out
 vx_status vx_scene_validate(const vx_scene_desc* desc);
 vx_status vx_scene_destroy(vx_scene* scene);
 
-/* Device addresses of a scene's pools (for callers that share them). */
+/* Descriptor/brick counts and tree shape of an uploaded scene. */
 vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* brick_count,
                         int32_t* user_max_levels, int32_t* bricks_per_axis);
 
@@ -196,6 +196,51 @@ vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t heigh
  * device pointer to 8 u64 that the call zeroes first. */
 vx_status vx_trace_counters(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits,
                             uint64_t* counters, vx_stream stream);
+
+/* Levels of the breadth-first descriptor array (0-4) that vx_trace/vx_render
+ * stage in LDS per workgroup (a prefix of the array, at most 64 KiB). */
+vx_status vx_scene_set_lds_depth(vx_scene* scene, uint32_t depth);
+
+/* ------------------------------------------------------------------------ */
+/* Batched multi-scene tracing (no reference counterpart: VIXEN dispatches   */
+/* one scene per compute pass, ComputeDispatchNode.cpp:249-317)              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vx_batch vx_batch;
+typedef struct vx_graph vx_graph;
+
+/* Launch shape of the persistent batched kernel. Zero fields take defaults. */
+typedef struct vx_tuning {
+    uint32_t waves_per_group; /* 1, 2, 4 or 8 wave64s per workgroup */
+    uint32_t chunk;           /* rays per queue chunk (a chunk never spans two scenes) */
+    uint32_t lds_depth;       /* breadth-first levels of every scene staged in LDS (0-4) */
+    uint32_t refill_min;      /* idle lanes that trigger a refill of a wave (1-64) */
+} vx_tuning;
+
+/* Concatenates the scenes' pools (device-to-device copies) with per-scene
+ * offsets. The scenes may be destroyed afterwards. */
+vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scenes, vx_batch** out);
+vx_status vx_batch_destroy(vx_batch* batch);
+
+/* Traces the rays of all scenes in one persistent launch: scene s owns
+ * rays[ray_offsets[s] .. ray_offsets[s+1]) and the same range of hits
+ * (ray_offsets is a HOST array of n_scenes + 1 entries). Results are
+ * bit-identical to vx_trace per scene. tuning NULL: the autotuned shape if
+ * vx_batch_autotune ran, else a default. Calls on one batch must not overlap. */
+vx_status vx_trace_batched(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                           const vx_tuning* tuning, vx_stream stream);
+
+/* Times every shape (waves {1,2,4,8} x chunk {32..1024} x LDS depth {0..3})
+ * on these rays, keeps the fastest for later calls and reports it. */
+vx_status vx_batch_autotune(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                            vx_tuning* best, float* best_ms, vx_stream stream);
+
+/* Captures one batched launch (queue reset + persistent kernel) in a hipGraph
+ * for per-frame replay on fixed buffers. */
+vx_status vx_batch_graph_create(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                                const vx_tuning* tuning, vx_graph** out);
+vx_status vx_graph_launch(vx_graph* graph, vx_stream stream);
+vx_status vx_graph_destroy(vx_graph* graph);
 
 #ifdef __cplusplus
 }

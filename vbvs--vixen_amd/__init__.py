@@ -79,6 +79,14 @@ class VxSceneDesc(C.Structure):
                 ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig))]
 
 
+class VxTuning(C.Structure):
+    _fields_ = [("waves_per_group", C.c_uint32), ("chunk", C.c_uint32), ("lds_depth", C.c_uint32),
+                ("refill_min", C.c_uint32)]
+
+    def to_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class VxBuiltScene(C.Structure):
     _fields_ = [("esvo_nodes", C.POINTER(C.c_uint32)), ("node_count", C.c_uint64),
                 ("brick_data", C.POINTER(C.c_uint32)), ("brick_count", C.c_uint64),
@@ -122,6 +130,14 @@ def _load() -> C.CDLL:
         "vx_build_from_dense": ([vp, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
+        "vx_scene_set_lds_depth": ([vp, u32], st),
+        "vx_batch_create": ([vp, C.POINTER(vp), u32, C.POINTER(vp)], st),
+        "vx_batch_destroy": ([vp], st),
+        "vx_trace_batched": ([vp, vp, C.POINTER(u64), vp, C.POINTER(VxTuning), vp], st),
+        "vx_batch_autotune": ([vp, vp, C.POINTER(u64), vp, C.POINTER(VxTuning), C.POINTER(C.c_float), vp], st),
+        "vx_batch_graph_create": ([vp, vp, C.POINTER(u64), vp, C.POINTER(VxTuning), C.POINTER(vp)], st),
+        "vx_graph_launch": ([vp, vp], st),
+        "vx_graph_destroy": ([vp], st),
         "vx_gen_cornell": ([u32, u64, vp, C.POINTER(VxCamera)], st),
         "vx_gen_cave": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
         "vx_gen_urban": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
@@ -139,7 +155,9 @@ LIB = _load()
 EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create", "vx_scene_validate",
             "vx_scene_destroy", "vx_scene_info", "vx_trace", "vx_trace_counters", "vx_render",
             "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
-            "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban")
+            "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
+            "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
+            "vx_graph_destroy")
 
 
 def _check(status: int, where: str) -> None:
@@ -325,6 +343,89 @@ class Scene:
         _check(LIB.vx_render(self._h, C.byref(cam), w, h, float(tmax), hb.data_ptr() if hb is not None else None,
                              ib.data_ptr() if ib is not None else None, _stream_ptr(stream)), "vx_render")
         return hb, ib
+
+
+    def set_lds_depth(self, depth: int) -> None:
+        _check(LIB.vx_scene_set_lds_depth(self._h, depth), "vx_scene_set_lds_depth")
+
+
+def _tuning(t) -> VxTuning | None:
+    if t is None or isinstance(t, VxTuning):
+        return t
+    return VxTuning(*[int(t.get(k, 0)) for k in ("waves_per_group", "chunk", "lds_depth", "refill_min")])
+
+
+class Batch:
+    """Many scenes in one persistent launch (concatenated pools, per-scene offsets)."""
+
+    def __init__(self, ctx: Context, scenes: list[Scene]):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(scenes))(*[sc._h.value for sc in scenes])
+        h = C.c_void_p()
+        _check(LIB.vx_batch_create(ctx._h, arr, len(scenes), C.byref(h)), "vx_batch_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _offsets(offsets) -> C.Array:
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        return (C.c_uint64 * len(o))(*[int(x) for x in o])
+
+    def trace(self, rays, offsets, hits=None, tuning=None, stream=None):
+        torch = _torch()
+        n = rays.numel() * rays.element_size() // 32
+        if hits is None:
+            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        t = _tuning(tuning)
+        _check(LIB.vx_trace_batched(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+                                    C.byref(t) if t is not None else None, _stream_ptr(stream)), "vx_trace_batched")
+        return hits
+
+    def autotune(self, rays, offsets, hits, stream=None):
+        best = VxTuning()
+        ms = C.c_float()
+        _check(LIB.vx_batch_autotune(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+                                     C.byref(best), C.byref(ms), _stream_ptr(stream)), "vx_batch_autotune")
+        return best, float(ms.value)
+
+    def graph(self, rays, offsets, hits, tuning=None) -> "Graph":
+        t = _tuning(tuning)
+        h = C.c_void_p()
+        _check(LIB.vx_batch_graph_create(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+                                         C.byref(t) if t is not None else None, C.byref(h)), "vx_batch_graph_create")
+        return Graph(h)
+
+
+class Graph:
+    """A captured batched launch (hipGraph) replayed per frame."""
+
+    def __init__(self, h: C.c_void_p):
+        self._h = h
+
+    def launch(self, stream=None) -> None:
+        _check(LIB.vx_graph_launch(self._h, _stream_ptr(stream)), "vx_graph_launch")
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None):

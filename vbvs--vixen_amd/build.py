@@ -39,7 +39,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp"]
-HIP_SOURCES = ["vx_api.hip"]
+HIP_SOURCES = ["vx_api.hip", "vx_batch.hip"]
 
 
 def _run(cmd: list[str]) -> None:

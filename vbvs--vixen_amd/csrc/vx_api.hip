@@ -1,10 +1,12 @@
 // vx_api.hip -- the C ABI of libvixen_hip.so: contexts, scene upload and the
 // trace/render launches for gfx950.
 //
-// Launch geometry (all kernels): 256-lane workgroups (4 x wave64), one ray per
+// Launch geometry (plain kernels): 256-lane workgroups (4 x wave64), one ray per
 // lane. Workgroup ids are remapped so that the 8 XCDs each own a contiguous
 // band of tiles/ray chunks (the hardware deals workgroup i to XCD i % 8), which
 // keeps neighbouring rays -- and the descriptors they share -- in one XCD's L2.
+// A prefix of the descriptor array (the top breadth-first levels) can be staged
+// in LDS per workgroup; vx_scene_set_lds_depth picks how many levels.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -13,9 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "vixen_hip.h"
-#include "vx_common.hpp"
-#include "vx_trace.hpp"
+#include "vx_internal.hpp"
 
 namespace vx {
 
@@ -28,15 +28,7 @@ vx_status fail(vx_status code, const std::string& msg)
 }
 void clear_error() { g_last_error.clear(); }
 
-#define VX_HIP(call)                                                                                 \
-    do {                                                                                             \
-        hipError_t e_ = (call);                                                                      \
-        if (e_ != hipSuccess)                                                                        \
-            return fail(VX_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_));      \
-    } while (0)
-
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kXcds = 8;
 
 struct CamParams {
     float pos[3], dir[3], up[3], right[3];
@@ -66,16 +58,16 @@ __global__ void __launch_bounds__(kBlock) k_build_occ(const uint32_t* __restrict
     occ[i] = m;
 }
 
-__device__ __forceinline__ void store_hit(vx_hit* __restrict__ out, uint64_t i, const vx_hit& h)
-{
-    uint4* dst = reinterpret_cast<uint4*>(out + i);
-    dst[0] = make_uint4(h.flags, __float_as_uint(h.t), h.leaf_node, h.brick);
-    dst[1] = make_uint4(h.voxel, __float_as_uint(h.rgb[0]), __float_as_uint(h.rgb[1]), __float_as_uint(h.rgb[2]));
-}
-
 __device__ __forceinline__ uint32_t xcd_chunk(uint32_t per_xcd)
 {
     return (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+}
+
+// Every lane of the workgroup copies part of the descriptor prefix into LDS.
+__device__ __forceinline__ void stage_prefix(const SceneView& s)
+{
+    for (uint32_t i = threadIdx.x; i < s.lds_count; i += blockDim.x) vx_lds_nodes[i] = s.nodes[i];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v)
@@ -91,14 +83,14 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
                                                   uint32_t per_xcd, vx_hit* __restrict__ hits,
                                                   unsigned long long* __restrict__ counters)
 {
+    stage_prefix(s);
     const uint64_t i = (uint64_t)xcd_chunk(per_xcd) * kBlock + threadIdx.x;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     if (i < n) {
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
-        const float4 a = rp[0], b = rp[1];
-        const float o[3] = {a.x, a.y, a.z}, d[3] = {b.x, b.y, b.z};
+        float o[3], d[3], tmin, tmax;
+        load_ray(rays, i, o, d, tmin, tmax);
         vx_hit h;
-        trace_ray<COUNT>(s, o, d, a.w, b.w, h, cnt);
+        trace_ray<COUNT>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
     if (COUNT) {
@@ -139,6 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uin
                                                    uint32_t tiles_x, uint32_t ntiles, uint32_t per_xcd,
                                                    vx_hit* __restrict__ hits, uint32_t* __restrict__ rgba)
 {
+    stage_prefix(s);
     const uint32_t tile = xcd_chunk(per_xcd);
     if (tile >= ntiles) return;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -207,26 +200,6 @@ static CamParams make_cam(const vx_camera* cam)
     return c;
 }
 
-} // namespace vx
-
-struct vx_ctx {
-    int device;
-};
-
-struct vx_scene {
-    vx_ctx* ctx;
-    uint2* nodes;
-    uint32_t* brick_data;
-    uint32_t* occ;
-    uint2* bc1;
-    uint64_t node_count;
-    uint64_t brick_count;
-    vx_octree_config cfg;
-    vx::SceneView view;
-};
-
-namespace vx {
-
 static int ilog2(uint32_t v)
 {
     int l = 0;
@@ -236,8 +209,9 @@ static int ilog2(uint32_t v)
 
 // Walks the tree from the root and checks every pointer the traversal will
 // follow: internal levels carry no leaf bits, the brick-parent level carries
-// leafMask == validMask, child blocks and brick ids are in range.
-static vx_status validate_tree(const vx_scene_desc* d, int levels)
+// leafMask == validMask, child blocks and brick ids are in range. Also counts
+// the descriptors of the top four levels (for LDS staging).
+static vx_status validate_tree(const vx_scene_desc* d, int levels, uint32_t level_prefix[4])
 {
     const uint32_t* n = d->esvo_nodes;
     const uint64_t nc = d->node_count;
@@ -245,6 +219,7 @@ static vx_status validate_tree(const vx_scene_desc* d, int levels)
         uint64_t idx;
         int depth;
     };
+    uint64_t per_depth[4] = {1, 0, 0, 0};
     std::vector<Item> todo;
     todo.push_back({0, 0});
     uint64_t visits = 0;
@@ -264,6 +239,7 @@ static vx_status validate_tree(const vx_scene_desc* d, int levels)
         const uint32_t cnt = static_cast<uint32_t>(__builtin_popcount(valid));
         if (cp + cnt > nc)
             return fail(VX_ERR_INVALID_SCENE, "child block of node " + std::to_string(it.idx) + " out of range");
+        if (it.depth + 1 < 4) per_depth[it.depth + 1] += cnt;
         for (uint32_t c = 0; c < cnt; ++c) {
             if (last) {
                 const uint32_t brick = n[2 * (cp + c) + 1] & 0xFFFFFFu;
@@ -274,10 +250,15 @@ static vx_status validate_tree(const vx_scene_desc* d, int levels)
             }
         }
     }
+    uint64_t acc = 0;
+    for (int k = 0; k < 4; ++k) {
+        acc += per_depth[k];
+        level_prefix[k] = static_cast<uint32_t>(acc < nc ? acc : nc);
+    }
     return VX_OK;
 }
 
-static vx_status scene_validate(const vx_scene_desc* d)
+static vx_status scene_validate_impl(const vx_scene_desc* d, uint32_t level_prefix[4])
 {
     if (!d || !d->config || !d->esvo_nodes) return fail(VX_ERR_INVALID_ARG, "vx_scene_validate: null argument");
     const vx_octree_config& c = *d->config;
@@ -294,14 +275,15 @@ static vx_status scene_validate(const vx_scene_desc* d)
     if (d->node_count < 1 || d->node_count > 0xFFFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "node_count out of range");
     if (d->brick_count >= 0xFFFFFFull) return fail(VX_ERR_INVALID_SCENE, "brick_count exceeds 24-bit ids");
     if (d->brick_count > 0 && !d->brick_data) return fail(VX_ERR_INVALID_ARG, "brick_data is null");
-    return validate_tree(d, levels);
+    return validate_tree(d, levels, level_prefix);
 }
 
 static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out)
 {
     if (!ctx || !d || !out) return fail(VX_ERR_INVALID_ARG, "vx_scene_create: null argument");
     *out = nullptr;
-    if (vx_status st = scene_validate(d); st != VX_OK) return st;
+    uint32_t level_prefix[4];
+    if (vx_status st = scene_validate_impl(d, level_prefix); st != VX_OK) return st;
     const vx_octree_config& c = *d->config;
     const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
     const float sx = c.grid_max[0] - c.grid_min[0];
@@ -316,6 +298,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     s->node_count = d->node_count;
     s->brick_count = d->brick_count;
     s->cfg = c;
+    std::memcpy(s->level_prefix, level_prefix, sizeof level_prefix);
     auto cleanup = [&](vx_status st) {
         (void)hipFree(s->nodes);
         (void)hipFree(s->brick_data);
@@ -354,9 +337,26 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.invN = 1.0f / v.Nf;
     v.Bf = static_cast<float>(B);
     v.min_scale = 22 - c.user_max_levels + 1;
+    v.lds_off = 0;
+    v.lds_count = 0;
+    v.stack_base = 0;
+    s->lds_depth = 2;
     *out = s;
     return VX_OK;
 }
+
+static SceneView launch_view(const vx_scene* s)
+{
+    SceneView v = s->view;
+    v.lds_off = 0;
+    v.lds_count = staged_nodes(s, s->lds_depth);
+    const uint32_t cap = (kMaxLdsBytes - stack_bytes(kBlock)) / 8u;
+    if (v.lds_count > cap) v.lds_count = cap;
+    v.stack_base = v.lds_count * 2u;
+    return v;
+}
+
+static size_t launch_lds(const SceneView& v) { return v.lds_count * sizeof(uint2) + stack_bytes(kBlock); }
 
 static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
                               hipStream_t st)
@@ -367,12 +367,14 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     const uint64_t per = (chunks + kXcds - 1) / kXcds;
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
+    const SceneView v = launch_view(s);
+    const size_t lds = launch_lds(v);
     if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
-        k_trace<true><<<grid, block, 0, st>>>(s->view, rays, n, static_cast<uint32_t>(per), hits,
-                                              reinterpret_cast<unsigned long long*>(counters));
+        k_trace<true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits,
+                                                reinterpret_cast<unsigned long long*>(counters));
     } else {
-        k_trace<false><<<grid, block, 0, st>>>(s->view, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        k_trace<false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
     }
     VX_HIP(hipGetLastError());
     return VX_OK;
@@ -384,7 +386,7 @@ extern "C" {
 
 const char* vx_last_error(void) { return vx::g_last_error.c_str(); }
 
-const char* vx_version(void) { return "vixen-hip 0.1 gfx950 wave64 strict-fp32"; }
+const char* vx_version(void) { return "vixen-hip 0.2 gfx950 wave64 strict-fp32"; }
 
 vx_status vx_ctx_create(int device, vx_ctx** out)
 {
@@ -398,7 +400,9 @@ vx_status vx_ctx_create(int device, vx_ctx** out)
         return vx::fail(VX_ERR_INVALID_ARG, "vx_ctx_create: device " + std::to_string(device) + " not in [0, " +
                                                 std::to_string(count) + ")");
     if (hipSetDevice(device) != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_ctx_create: hipSetDevice failed");
-    auto* c = new (std::nothrow) vx_ctx{device};
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    auto* c = new (std::nothrow) vx_ctx{device, cus};
     if (!c) return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_ctx_create: out of memory");
     *out = c;
     return VX_OK;
@@ -424,7 +428,8 @@ vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out
 vx_status vx_scene_validate(const vx_scene_desc* desc)
 {
     try {
-        return vx::scene_validate(desc);
+        uint32_t level_prefix[4];
+        return vx::scene_validate_impl(desc, level_prefix);
     } catch (...) {
         return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_validate: unexpected exception");
     }
@@ -452,6 +457,13 @@ vx_status vx_scene_info(const vx_scene* s, uint64_t* node_count, uint64_t* brick
     return VX_OK;
 }
 
+vx_status vx_scene_set_lds_depth(vx_scene* s, uint32_t depth)
+{
+    if (!s || depth > 4) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_set_lds_depth: depth must be 0..4");
+    s->lds_depth = depth;
+    return VX_OK;
+}
+
 vx_status vx_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, vx_stream stream)
 {
     return vx::launch_trace(s, rays, n, hits, nullptr, static_cast<hipStream_t>(stream));
@@ -474,8 +486,9 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    vx::k_render<<<dim3(per * vx::kXcds), dim3(vx::kBlock), 0, st>>>(s->view, c, w, h, tmax, tiles_x, ntiles, per,
-                                                                     hits, rgba8);
+    const vx::SceneView v = vx::launch_view(s);
+    vx::k_render<<<dim3(per * vx::kXcds), dim3(vx::kBlock), vx::launch_lds(v), st>>>(
+        v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
     return VX_OK;
 }

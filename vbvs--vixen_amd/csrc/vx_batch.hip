@@ -1,0 +1,482 @@
+// vx_batch.hip -- many scenes of uneven size traced in one persistent launch.
+//
+// The reference has no counterpart: VIXEN dispatches one scene per compute
+// pass (libraries/RenderGraph/src/Nodes/ComputeDispatchNode.cpp:249-317).
+// Batching changes dispatch and memory, never arithmetic: every ray runs the
+// same init_ray/step_ray code as vx_trace, so results are bit-identical to
+// per-scene launches.
+//
+// Memory: the scenes' descriptor, brick, occupancy and BC1 pools are copied
+// into four concatenated device pools; each scene's SceneView points at its
+// own slice (per-scene offsets), so descriptor and brick indices stay
+// scene-relative and hit records match the per-scene ones byte for byte.
+//
+// Dispatch: the rays of scene s are rays[off[s] .. off[s+1]); they are cut into
+// chunks of `chunk` rays that never straddle two scenes, numbered scene by
+// scene. A persistent grid (occupancy-sized) pulls chunk numbers from one
+// global atomic counter. Each wave keeps 64 rays in flight: between ESVO
+// iterations it ballots its retired lanes and, once at least `refill_min` are
+// idle, hands them the next rays of its chunk (ranked by mbcnt), so divergent
+// rays do not leave lanes idle. A wave works on one scene at a time; when the
+// next chunk belongs to another scene it drains its lanes first, which keeps
+// the scene constants wave-uniform.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "vx_internal.hpp"
+
+namespace vx {
+
+struct BatchSceneInfo {
+    SceneView view;       // pointers into the concatenated pools
+    uint64_t ray_begin;   // first ray of this scene
+    uint64_t ray_count;
+    uint64_t chunk_begin; // first chunk number of this scene
+};
+
+constexpr uint32_t kMaxGroup = 512;
+constexpr uint64_t kNoChunk = ~0ull;
+
+__device__ __forceinline__ uint32_t find_scene(const BatchSceneInfo* __restrict__ sc, uint32_t n, uint64_t c)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (sc[mid].chunk_begin <= c) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_reset_queue(unsigned int* q) { *q = 0u; }
+
+__global__ void __launch_bounds__(kMaxGroup)
+    k_trace_persistent(const BatchSceneInfo* __restrict__ scenes, uint32_t n_scenes, uint64_t n_chunks,
+                       uint32_t chunk, uint32_t refill_min, uint32_t stack_base, const vx_ray* __restrict__ rays,
+                       vx_hit* __restrict__ hits, unsigned int* __restrict__ queue)
+{
+    // stage every scene's descriptor prefix (lds_off/lds_count set per launch)
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        const uint2* src = scenes[s].view.nodes;
+        const uint32_t off = scenes[s].view.lds_off, cnt = scenes[s].view.lds_count;
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) vx_lds_nodes[off + i] = src[i];
+    }
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t full = ~0ull;
+    uint64_t cur = 0, end = 0, pending = kNoChunk;
+    int scene = -1;
+    bool drained = false;
+    SceneView sv = scenes[0].view;
+    sv.stack_base = stack_base;
+
+    bool active = false;
+    uint64_t ray_id = 0;
+    RayState r;
+    vx_hit h;
+    TraceCounts cnt{0u, 0u, 0u, 0u};
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if ((uint32_t)__popcll(idle) >= refill_min || idle == full) {
+            while (idle != 0ull) {
+                if (cur == end) {
+                    uint64_t c;
+                    if (pending != kNoChunk) {
+                        c = pending;
+                    } else if (drained) {
+                        break;
+                    } else {
+                        unsigned int v = 0u;
+                        if (lane == 0u) v = atomicAdd(queue, 1u);
+                        c = (uint64_t)__shfl(v, 0, 64);
+                        if (c >= n_chunks) {
+                            drained = true;
+                            break;
+                        }
+                    }
+                    const int cs = (int)find_scene(scenes, n_scenes, c);
+                    if (cs != scene && idle != full) {
+                        pending = c; // another scene: drain the wave first
+                        break;
+                    }
+                    pending = kNoChunk;
+                    if (cs != scene) {
+                        scene = cs;
+                        sv = scenes[cs].view;
+                        sv.stack_base = stack_base;
+                    }
+                    const uint64_t rb = scenes[cs].ray_begin, rc = scenes[cs].ray_count;
+                    cur = rb + (c - scenes[cs].chunk_begin) * chunk;
+                    end = min(cur + chunk, rb + rc);
+                }
+                const uint64_t take = min((uint64_t)__popcll(idle), end - cur);
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (!active && rank < take) {
+                    ray_id = cur + rank;
+                    float o[3], d[3], tmin, tmax;
+                    load_ray(rays, ray_id, o, d, tmin, tmax);
+                    if (init_ray(sv, o, d, tmin, tmax, r, h)) store_hit(hits, ray_id, h);
+                    else active = true;
+                }
+                cur += take;
+                idle = __ballot(!active);
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained && pending == kNoChunk && cur == end) break;
+            continue;
+        }
+        if (active && step_ray<false>(sv, r, h, cnt)) {
+            store_hit(hits, ray_id, h);
+            active = false;
+        }
+    }
+}
+
+} // namespace vx
+
+struct vx_batch {
+    vx_ctx* ctx;
+    uint32_t n;
+    std::vector<vx::SceneView> views;     // slices of the concatenated pools
+    std::vector<uint32_t> level_prefix;   // n x 4
+    uint2* nodes = nullptr;
+    uint32_t* brick_data = nullptr;
+    uint32_t* occ = nullptr;
+    uint2* bc1 = nullptr;
+    // launch state of vx_trace_batched
+    vx::BatchSceneInfo* d_table = nullptr;
+    unsigned int* d_queue = nullptr;
+    std::vector<vx::BatchSceneInfo> h_table;
+    vx_tuning tuned{0, 0, 0, 0};
+};
+
+struct vx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    vx::BatchSceneInfo* d_table = nullptr;
+    unsigned int* d_queue = nullptr;
+};
+
+namespace vx {
+
+static vx_tuning default_tuning() { return vx_tuning{4, 128, 2, 32}; }
+
+static vx_tuning sanitize(const vx_tuning* t, const vx_batch* b)
+{
+    vx_tuning r = t ? *t : (b->tuned.chunk ? b->tuned : default_tuning());
+    if (r.waves_per_group == 0 || r.waves_per_group > kMaxGroup / 64) r.waves_per_group = 4;
+    if (r.chunk == 0) r.chunk = 128;
+    if (r.lds_depth > 4) r.lds_depth = 4;
+    if (r.refill_min == 0 || r.refill_min > 64) r.refill_min = 32;
+    return r;
+}
+
+// Builds the per-launch scene table; returns the LDS bytes it needs.
+static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tuning& t,
+                            std::vector<BatchSceneInfo>& tab, uint64_t& n_chunks, uint32_t& lds_bytes)
+{
+    tab.resize(b->n);
+    uint64_t chunk_acc = 0;
+    uint32_t lds_acc = 0;
+    for (uint32_t s = 0; s < b->n; ++s) {
+        if (off[s + 1] < off[s]) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: ray_offsets must be non-decreasing");
+        BatchSceneInfo& e = tab[s];
+        e.view = b->views[s];
+        e.ray_begin = off[s];
+        e.ray_count = off[s + 1] - off[s];
+        e.chunk_begin = chunk_acc;
+        chunk_acc += (e.ray_count + t.chunk - 1) / t.chunk;
+        uint32_t want = 0;
+        if (t.lds_depth) want = b->level_prefix[4 * s + std::min<uint32_t>(t.lds_depth, 4) - 1];
+        if ((lds_acc + want) * 8u + stack_bytes(t.waves_per_group * 64) > kMaxLdsBytes) want = 0;
+        e.view.lds_off = lds_acc;
+        e.view.lds_count = want;
+        lds_acc += want;
+    }
+    n_chunks = chunk_acc;
+    lds_bytes = lds_acc * 8u;
+    return VX_OK;
+}
+
+static bool same_table(const std::vector<BatchSceneInfo>& a, const std::vector<BatchSceneInfo>& b)
+{
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * sizeof(BatchSceneInfo)) == 0;
+}
+
+static vx_status grid_size(const vx_batch* b, const vx_tuning& t, uint32_t lds_bytes, unsigned& groups)
+{
+    const int block = static_cast<int>(t.waves_per_group * 64);
+    int per_cu = 0;
+    VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent, block, lds_bytes));
+    if (per_cu < 1) return fail(VX_ERR_UNSUPPORTED, "persistent kernel does not fit one workgroup per CU");
+    groups = static_cast<unsigned>(per_cu * b->ctx->cu_count);
+    return VX_OK;
+}
+
+static vx_status enqueue(vx_batch* b, BatchSceneInfo* d_table, unsigned int* d_queue, uint64_t n_chunks,
+                         const vx_tuning& t, uint32_t lds_bytes, const vx_ray* rays, vx_hit* hits, hipStream_t st)
+{
+    unsigned groups = 0;
+    const uint32_t total_lds = lds_bytes + stack_bytes(t.waves_per_group * 64);
+    if (vx_status s = grid_size(b, t, total_lds, groups); s != VX_OK) return s;
+    k_reset_queue<<<1, 1, 0, st>>>(d_queue);
+    if (n_chunks)
+        k_trace_persistent<<<groups, t.waves_per_group * 64, total_lds, st>>>(
+            d_table, b->n, n_chunks, t.chunk, t.refill_min, lds_bytes / 4u, rays, hits, d_queue);
+    VX_HIP(hipGetLastError());
+    return VX_OK;
+}
+
+static vx_status trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* off, vx_hit* hits,
+                               const vx_tuning* tuning, hipStream_t st)
+{
+    if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null argument");
+    if (off[b->n] > off[0] && (!rays || !hits)) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null rays/hits");
+    const vx_tuning t = sanitize(tuning, b);
+    std::vector<BatchSceneInfo> tab;
+    uint64_t n_chunks = 0;
+    uint32_t lds_bytes = 0;
+    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes); s != VX_OK) return s;
+    if (n_chunks > 0xFFFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: more than 2^32 chunks");
+    if (!same_table(tab, b->h_table)) {
+        // the device table may still be read by an earlier launch on any stream
+        VX_HIP(hipDeviceSynchronize());
+        VX_HIP(hipMemcpy(b->d_table, tab.data(), tab.size() * sizeof(BatchSceneInfo), hipMemcpyHostToDevice));
+        b->h_table = tab;
+    }
+    return enqueue(b, b->d_table, b->d_queue, n_chunks, t, lds_bytes, rays, hits, st);
+}
+
+static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, vx_batch** out)
+{
+    if (!ctx || !scenes || !out || n == 0) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: bad arguments");
+    *out = nullptr;
+    uint64_t nodes = 0, bricks = 0, bc1_bricks = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        if (!scenes[s]) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: null scene");
+        nodes += scenes[s]->node_count;
+        bricks += scenes[s]->brick_count;
+        if (scenes[s]->view.bc1) bc1_bricks += scenes[s]->brick_count;
+    }
+    auto* b = new (std::nothrow) vx_batch{};
+    if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: out of memory");
+    b->ctx = ctx;
+    b->n = n;
+    auto cleanup = [&](vx_status st) {
+        (void)hipFree(b->nodes);
+        (void)hipFree(b->brick_data);
+        (void)hipFree(b->occ);
+        (void)hipFree(b->bc1);
+        (void)hipFree(b->d_table);
+        (void)hipFree(b->d_queue);
+        delete b;
+        return st;
+    };
+    if (hipMalloc(&b->nodes, nodes * 8) != hipSuccess ||
+        hipMalloc(&b->brick_data, std::max<uint64_t>(1, bricks) * 2048) != hipSuccess ||
+        hipMalloc(&b->occ, std::max<uint64_t>(1, bricks) * 64) != hipSuccess ||
+        hipMalloc(&b->bc1, std::max<uint64_t>(1, bc1_bricks) * 256) != hipSuccess ||
+        hipMalloc(&b->d_table, n * sizeof(BatchSceneInfo)) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess)
+        return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: device allocation failed"));
+    uint64_t nb = 0, bb = 0, cb = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        const vx_scene* sc = scenes[s];
+        if (hipMemcpy(b->nodes + nb, sc->nodes, sc->node_count * 8, hipMemcpyDeviceToDevice) != hipSuccess ||
+            (sc->brick_count &&
+             (hipMemcpy(b->brick_data + bb * 512, sc->brick_data, sc->brick_count * 2048, hipMemcpyDeviceToDevice) !=
+                  hipSuccess ||
+              hipMemcpy(b->occ + bb * 16, sc->occ, sc->brick_count * 64, hipMemcpyDeviceToDevice) != hipSuccess)) ||
+            (sc->view.bc1 && sc->brick_count &&
+             hipMemcpy(b->bc1 + cb * 32, sc->bc1, sc->brick_count * 256, hipMemcpyDeviceToDevice) != hipSuccess))
+            return cleanup(fail(VX_ERR_HIP, "vx_batch_create: pool copy failed"));
+        SceneView v = sc->view;
+        v.nodes = b->nodes + nb;
+        v.brick_data = b->brick_data + bb * 512;
+        v.occ = b->occ + bb * 16;
+        v.bc1 = sc->view.bc1 ? b->bc1 + cb * 32 : nullptr;
+        v.lds_off = 0;
+        v.lds_count = 0;
+        b->views.push_back(v);
+        for (int k = 0; k < 4; ++k) b->level_prefix.push_back(sc->level_prefix[k]);
+        nb += sc->node_count;
+        bb += sc->brick_count;
+        if (sc->view.bc1) cb += sc->brick_count;
+    }
+    *out = b;
+    return VX_OK;
+}
+
+static vx_status autotune(vx_batch* b, const vx_ray* rays, const uint64_t* off, vx_hit* hits, vx_tuning* best,
+                          float* best_ms, hipStream_t st)
+{
+    if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_batch_autotune: null argument");
+    hipEvent_t e0, e1;
+    VX_HIP(hipEventCreate(&e0));
+    VX_HIP(hipEventCreate(&e1));
+    const uint32_t waves[] = {1, 2, 4, 8};
+    const uint32_t chunks[] = {32, 64, 128, 256, 512, 1024};
+    vx_tuning bt = default_tuning();
+    float bms = 1e30f;
+    vx_status status = VX_OK;
+    for (uint32_t depth = 0; depth <= 3 && status == VX_OK; ++depth)
+        for (uint32_t w : waves)
+            for (uint32_t c : chunks) {
+                const vx_tuning t{w, c, depth, 32};
+                std::vector<BatchSceneInfo> tab;
+                uint64_t n_chunks = 0;
+                uint32_t lds_bytes = 0;
+                if ((status = make_table(b, off, t, tab, n_chunks, lds_bytes)) != VX_OK) break;
+                if (depth > 0 && lds_bytes == 0) continue; // nothing staged: same as depth 0
+                float ms = 1e30f;
+                for (int rep = 0; rep < 3 && status == VX_OK; ++rep) {
+                    if (hipEventRecord(e0, st) != hipSuccess) status = fail(VX_ERR_HIP, "event record failed");
+                    if (status == VX_OK) status = trace_batched(b, rays, off, hits, &t, st);
+                    if (status == VX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                        status = fail(VX_ERR_HIP, "event sync failed");
+                    float m = 0.0f;
+                    if (status == VX_OK && hipEventElapsedTime(&m, e0, e1) == hipSuccess && rep > 0) ms = std::min(ms, m);
+                }
+                if (status == VX_OK && ms < bms) {
+                    bms = ms;
+                    bt = t;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (status != VX_OK) return status;
+    b->tuned = bt;
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
+    return VX_OK;
+}
+
+static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* off, vx_hit* hits,
+                              const vx_tuning* tuning, vx_graph** out)
+{
+    if (!b || !off || !out) return fail(VX_ERR_INVALID_ARG, "vx_batch_graph_create: null argument");
+    *out = nullptr;
+    const vx_tuning t = sanitize(tuning, b);
+    std::vector<BatchSceneInfo> tab;
+    uint64_t n_chunks = 0;
+    uint32_t lds_bytes = 0;
+    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes); s != VX_OK) return s;
+    auto* g = new (std::nothrow) vx_graph{};
+    if (!g) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: out of memory");
+    auto cleanup = [&](vx_status st) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        (void)hipFree(g->d_table);
+        (void)hipFree(g->d_queue);
+        delete g;
+        return st;
+    };
+    if (hipMalloc(&g->d_table, tab.size() * sizeof(BatchSceneInfo)) != hipSuccess ||
+        hipMalloc(&g->d_queue, sizeof(unsigned int)) != hipSuccess)
+        return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: device allocation failed"));
+    if (hipMemcpy(g->d_table, tab.data(), tab.size() * sizeof(BatchSceneInfo), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(VX_ERR_HIP, "vx_batch_graph_create: table upload failed"));
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(VX_ERR_HIP, "vx_batch_graph_create: stream creation failed"));
+    vx_status st = VX_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        st = fail(VX_ERR_HIP, "vx_batch_graph_create: begin capture failed");
+    } else {
+        st = enqueue(b, g->d_table, g->d_queue, n_chunks, t, lds_bytes, rays, hits, cs);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        g->graph = graph;
+        if (st == VX_OK && e != hipSuccess) st = fail(VX_ERR_HIP, "vx_batch_graph_create: end capture failed");
+        if (st == VX_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+            st = fail(VX_ERR_HIP, "vx_batch_graph_create: instantiate failed");
+    }
+    (void)hipStreamDestroy(cs);
+    if (st != VX_OK) return cleanup(st);
+    *out = g;
+    return VX_OK;
+}
+
+} // namespace vx
+
+extern "C" {
+
+vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scenes, vx_batch** out)
+{
+    try {
+        return vx::batch_create(ctx, scenes, n_scenes, out);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: out of memory");
+    }
+}
+
+vx_status vx_batch_destroy(vx_batch* b)
+{
+    if (!b) return VX_OK;
+    (void)hipFree(b->nodes);
+    (void)hipFree(b->brick_data);
+    (void)hipFree(b->occ);
+    (void)hipFree(b->bc1);
+    (void)hipFree(b->d_table);
+    (void)hipFree(b->d_queue);
+    delete b;
+    return VX_OK;
+}
+
+vx_status vx_trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                           const vx_tuning* tuning, vx_stream stream)
+{
+    try {
+        return vx::trace_batched(b, rays, ray_offsets, hits, tuning, static_cast<hipStream_t>(stream));
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_trace_batched: out of memory");
+    }
+}
+
+vx_status vx_batch_autotune(vx_batch* b, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                            vx_tuning* best, float* best_ms, vx_stream stream)
+{
+    try {
+        return vx::autotune(b, rays, ray_offsets, hits, best, best_ms, static_cast<hipStream_t>(stream));
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_autotune: out of memory");
+    }
+}
+
+vx_status vx_batch_graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
+                                const vx_tuning* tuning, vx_graph** out)
+{
+    try {
+        return vx::graph_create(b, rays, ray_offsets, hits, tuning, out);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: out of memory");
+    }
+}
+
+vx_status vx_graph_launch(vx_graph* g, vx_stream stream)
+{
+    if (!g || !g->exec) return vx::fail(VX_ERR_INVALID_ARG, "vx_graph_launch: null graph");
+    if (hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) != hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_graph_launch: hipGraphLaunch failed");
+    return VX_OK;
+}
+
+vx_status vx_graph_destroy(vx_graph* g)
+{
+    if (!g) return VX_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->d_table);
+    (void)hipFree(g->d_queue);
+    delete g;
+    return VX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,55 @@
+// vx_internal.hpp -- library-private definitions shared by the HIP translation
+// units of libvixen_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "vixen_hip.h"
+#include "vx_common.hpp"
+#include "vx_trace.hpp"
+
+struct vx_ctx {
+    int device;
+    int cu_count;
+};
+
+struct vx_scene {
+    vx_ctx* ctx;
+    uint2* nodes;
+    uint32_t* brick_data;
+    uint32_t* occ;
+    uint2* bc1;
+    uint64_t node_count;
+    uint64_t brick_count;
+    vx_octree_config cfg;
+    vx::SceneView view;
+    // descriptors in the first k+1 breadth-first levels, k = 0..3 (a prefix of
+    // the descriptor array when it is in breadth-first order)
+    uint32_t level_prefix[4];
+    uint32_t lds_depth; // levels staged in LDS by vx_trace / vx_render (0 = none)
+};
+
+namespace vx {
+
+#define VX_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return ::vx::fail(VX_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kMaxLdsBytes = 65536;
+
+// Staged descriptor count for `depth` levels of `s` (0 = none).
+inline uint32_t staged_nodes(const vx_scene* s, uint32_t depth)
+{
+    if (depth == 0) return 0;
+    const uint32_t d = depth > 4 ? 4 : depth;
+    return s->level_prefix[d - 1];
+}
+
+} // namespace vx

@@ -16,7 +16,9 @@
 // Layout choices that are this backend's own:
 //   - the traversal stack only holds the scales a well-formed tree can reach
 //     (22 down to one above the brick level, at most VX_STACK entries) and lives
-//     in VGPRs: every access is a fully unrolled compare/select, never scratch;
+//     in LDS, one lane-strided column per lane (bank-conflict free, no VGPRs,
+//     no scratch); every entry a ray reads was written earlier on its own path,
+//     so the stack needs no initialisation;
 //   - brick occupancy is read from a 64-byte-per-brick bit mask derived from
 //     brickData at upload (bit lin of the brick's 16 words), so a brick's DDA
 //     touches one cache line instead of 2 KiB; the hit voxel's brickData word
@@ -49,6 +51,9 @@ struct SceneView {
     float invN;       // 1 / N (a power of two)
     float Bf;         // bricks per axis
     int min_scale;
+    uint32_t lds_off;    // first LDS slot of this scene's staged descriptors
+    uint32_t lds_count;  // descriptors [0, lds_count) are read from LDS
+    uint32_t stack_base; // first LDS word of the per-lane traversal stacks
 };
 
 struct TraceCounts {
@@ -69,6 +74,28 @@ __device__ __forceinline__ void store_miss(vx_hit& h, uint32_t iters)
     h.brick = 0xFFFFFFFFu;
     h.voxel = 0xFFFFFFFFu;
     h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
+}
+
+__device__ __forceinline__ void load_ray(const vx_ray* __restrict__ rays, uint64_t i, float o[3], float d[3],
+                                         float& tmin, float& tmax)
+{
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    tmin = a.w;
+    d[0] = b.x;
+    d[1] = b.y;
+    d[2] = b.z;
+    tmax = b.w;
+}
+
+__device__ __forceinline__ void store_hit(vx_hit* __restrict__ out, uint64_t i, const vx_hit& h)
+{
+    uint4* dst = reinterpret_cast<uint4*>(out + i);
+    dst[0] = make_uint4(h.flags, __float_as_uint(h.t), h.leaf_node, h.brick);
+    dst[1] = make_uint4(h.voxel, __float_as_uint(h.rgb[0]), __float_as_uint(h.rgb[1]), __float_as_uint(h.rgb[2]));
 }
 
 // BC1 decode of one texel (integer extraction, u32->f32 round-to-nearest, x 2^-32,
@@ -148,15 +175,6 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
     return true;
 }
 
-__device__ __forceinline__ float corrected_tc_max(float tx, float ty, float tz, const float du[3], float t_max)
-{
-    const float thr = 1000.0f;
-    const float vx = (__builtin_fabsf(du[0]) >= DIR_EPS && __builtin_fabsf(tx) < thr) ? tx : t_max;
-    const float vy = (__builtin_fabsf(du[1]) >= DIR_EPS && __builtin_fabsf(ty) < thr) ? ty : t_max;
-    const float vz = (__builtin_fabsf(du[2]) >= DIR_EPS && __builtin_fabsf(tz) < thr) ? tz : t_max;
-    return mn(mn(vx, vy), vz);
-}
-
 // Brick DDA in local voxel space. Returns true on a hit and fills the hit record.
 template <bool COUNT>
 __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, const int bc[3], const float ol[3],
@@ -228,12 +246,20 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
             }
             float tHit = mx(mx(tnh[0], tnh[1]), tnh[2]);
             tHit = mx(tHit, tstart);
-            uint32_t axis;
-            if (tnh[0] >= tnh[1] && tnh[0] >= tnh[2]) axis = 0u;
-            else if (tnh[1] >= tnh[2]) axis = 1u;
-            else axis = 2u;
-            const float da = axis == 0u ? dl[0] : (axis == 1u ? dl[1] : dl[2]);
-            const uint32_t signpos = (da > 0.0f) ? 0u : 1u;
+            // sign bits computed for all axes and selected as values: an indexed
+            // read of dl[] here would make the ray state addressable (scratch)
+            const uint32_t sx = dl[0] > 0.0f ? 0u : 1u, sy = dl[1] > 0.0f ? 0u : 1u, sz = dl[2] > 0.0f ? 0u : 1u;
+            uint32_t axis, signpos;
+            if (tnh[0] >= tnh[1] && tnh[0] >= tnh[2]) {
+                axis = 0u;
+                signpos = sx;
+            } else if (tnh[1] >= tnh[2]) {
+                axis = 1u;
+                signpos = sy;
+            } else {
+                axis = 2u;
+                signpos = sz;
+            }
             h.flags = 1u | (axis << 1) | (signpos << 3);
             h.t = tHit;
             h.brick = brick;
@@ -259,220 +285,261 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     return false;
 }
 
-// VGPR-resident stack indexed by k = 22 - scale; every access is unrolled so
-// the arrays never spill to scratch.
-struct RegStack {
-    uint32_t ptr[VX_STACK];
-    float tmax[VX_STACK];
-    __device__ __forceinline__ void fill(float t)
-    {
-#pragma unroll
-        for (int i = 0; i < VX_STACK; ++i) {
-            ptr[i] = 0u;
-            tmax[i] = t;
-        }
-    }
-    __device__ __forceinline__ void put(int k, uint32_t p, float t)
-    {
-#pragma unroll
-        for (int i = 0; i < VX_STACK; ++i)
-            if (i == k) {
-                ptr[i] = p;
-                tmax[i] = t;
-            }
-    }
-    __device__ __forceinline__ float get_tmax(int k) const
-    {
-        float r = tmax[0];
-#pragma unroll
-        for (int i = 1; i < VX_STACK; ++i)
-            if (i == k) r = tmax[i];
-        return r;
-    }
-    __device__ __forceinline__ uint32_t get_ptr(int k) const
-    {
-        uint32_t r = ptr[0];
-#pragma unroll
-        for (int i = 1; i < VX_STACK; ++i)
-            if (i == k) r = ptr[i];
-        return r;
-    }
+// Descriptor fetch: a prefix of the descriptor array (the top levels of a
+// breadth-first tree) may be staged in LDS by the launching kernel.
+// The launching kernel's dynamic LDS: staged descriptors first, then the
+// traversal stacks (VX_STACK parent pointers and VX_STACK t_max words per lane,
+// word (level * blockDim.x + threadIdx.x) of each half).
+extern __shared__ uint2 vx_lds_nodes[];
+
+__device__ __forceinline__ uint2 fetch_node(const SceneView& s, uint32_t i)
+{
+    if (i < s.lds_count) return vx_lds_nodes[s.lds_off + i];
+    return s.nodes[i];
+}
+
+__device__ __forceinline__ uint32_t* stack_word(const SceneView& s, int k, bool tmax)
+{
+    uint32_t* words = reinterpret_cast<uint32_t*>(vx_lds_nodes);
+    return words + s.stack_base + ((tmax ? VX_STACK : 0) + k) * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ void stack_put(const SceneView& s, int k, uint32_t ptr, float t)
+{
+    *stack_word(s, k, false) = ptr;
+    *stack_word(s, k, true) = f2u(t);
+}
+__device__ __forceinline__ float stack_tmax(const SceneView& s, int k) { return u2f(*stack_word(s, k, true)); }
+__device__ __forceinline__ uint32_t stack_ptr(const SceneView& s, int k) { return *stack_word(s, k, false); }
+
+// LDS bytes a kernel needs for `lanes` traversal stacks.
+constexpr uint32_t stack_bytes(uint32_t lanes) { return lanes * VX_STACK * 2u * 4u; }
+
+// Per-lane traversal state between iterations of the ESVO loop. Holding it in
+// a struct lets a kernel suspend a ray after any iteration (persistent kernels
+// refill retired lanes between iterations).
+struct RayState {
+    float ol[3], dl[3];      // local-space ray (voxel units)
+    float tstart, texit;     // traced segment
+    float tc[3], tb[3];      // ESVO coefficients
+    float pos[3];
+    float scale_exp2, t_min, t_max;
+    uint32_t parent;
+    int idx, scale, iter;
+    int mask;                // octant mask (bits 0-2) | can-step flags (bits 3-5)
 };
 
-// Full traversal of one ray. Writes the hit record; returns true on a hit.
-template <bool COUNT>
-__device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
-                                          float tmax, vx_hit& h, TraceCounts& cnt)
+// Ray set-up, coefficients and initTraversalState. Returns true when the ray
+// is already finished (a miss written to h); false when the loop must run.
+__device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
+                                         float tmax, RayState& r, vx_hit& h)
 {
     store_miss(h, 0u);
-    float ol[3], dl[3], du[3], tstart, texit;
-    if (!setup_ray(s, o, d, tmin, tmax, ol, dl, du, tstart, texit)) return false;
+    float du[3];
+    if (!setup_ray(s, o, d, tmin, tmax, r.ol, r.dl, du, r.tstart, r.texit)) return true;
 
     float n0[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) n0[i] = (ol[i] + dl[i] * tstart) * s.invN + 1.0f;
+    for (int i = 0; i < 3; ++i) n0[i] = (r.ol[i] + r.dl[i] * r.tstart) * s.invN + 1.0f;
 
-    float tc[3], tb[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float ds = du[i];
         if (__builtin_fabsf(ds) < DIR_EPS) ds = __builtin_copysignf(DIR_EPS, ds);
-        tc[i] = 1.0f / -__builtin_fabsf(ds);
-        tb[i] = tc[i] * n0[i];
+        r.tc[i] = 1.0f / -__builtin_fabsf(ds);
+        r.tb[i] = r.tc[i] * n0[i];
     }
     int mask = 7;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         if (du[i] > 0.0f) {
             mask ^= 1 << i;
-            tb[i] = 3.0f * tc[i] - tb[i];
+            r.tb[i] = 3.0f * r.tc[i] - r.tb[i];
         }
 
-    float t_min = mx(mx(2.0f * tc[0] - tb[0], 2.0f * tc[1] - tb[1]), 2.0f * tc[2] - tb[2]);
-    float t_max = mn(mn(tc[0] - tb[0], tc[1] - tb[1]), tc[2] - tb[2]);
+    float t_min = mx(mx(2.0f * r.tc[0] - r.tb[0], 2.0f * r.tc[1] - r.tb[1]), 2.0f * r.tc[2] - r.tb[2]);
+    const float t_max = mn(mn(r.tc[0] - r.tb[0], r.tc[1] - r.tb[1]), r.tc[2] - r.tb[2]);
     t_min = mx(t_min, 0.0f);
-    if (t_min >= t_max) return false;
+    if (t_min >= t_max) return true;
 
-    RegStack stk;
-    stk.fill(t_max);
-    uint32_t parent = 0u;
-    int scale = ESVO_MAX_SCALE;
-    float scale_exp2 = 0.5f;
-    float pos[3] = {1.0f, 1.0f, 1.0f};
+    r.parent = 0u;
+    r.scale = ESVO_MAX_SCALE;
+    r.scale_exp2 = 0.5f;
+    r.iter = 0;
     int idx = 0;
     const bool use_pos = t_min < 1e-4f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
+        r.pos[i] = 1.0f;
         const float m = (mask & (1 << i)) ? n0[i] : (3.0f - n0[i]);
         if (__builtin_fabsf(du[i]) < DIR_EPS || use_pos) {
             if (m >= 1.5f) {
                 idx |= 1 << i;
-                pos[i] = 1.5f;
+                r.pos[i] = 1.5f;
             }
-        } else if (1.5f * tc[i] - tb[i] > t_min) {
+        } else if (1.5f * r.tc[i] - r.tb[i] > t_min) {
             idx ^= 1 << i;
-            pos[i] = 1.5f;
+            r.pos[i] = 1.5f;
         }
     }
-    const bool canx = __builtin_fabsf(du[0]) >= DIR_EPS, cany = __builtin_fabsf(du[1]) >= DIR_EPS,
-               canz = __builtin_fabsf(du[2]) >= DIR_EPS;
-
-    int iter;
-    for (iter = 0; iter < MAX_ITERS && scale <= ESVO_MAX_SCALE; ++iter) {
-        if (COUNT) cnt.iters += 1;
-        if (parent >= s.node_count) {
-            store_miss(h, (uint32_t)(iter + 1));
-            return false;
-        }
-        const uint2 desc = s.nodes[parent];
-        const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
-        const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
-        const int local = idx ^ ((~mask) & 7);
-
-        bool process = false;
-        float tv_max = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        if (((valid >> local) & 1u) && !(t_min > t_max + T_EPS)) {
-            const float tx = pos[0] * tc[0] - tb[0];
-            const float ty = pos[1] * tc[1] - tb[1];
-            const float tz = pos[2] * tc[2] - tb[2];
-            const float tcm = corrected_tc_max(tx, ty, tz, du, t_max);
-            tv_max = mn(t_max, tcm);
-            const float half = scale_exp2 * 0.5f;
-            cx = half * tc[0] + tx;
-            cy = half * tc[1] + ty;
-            cz = half * tc[2] + tz;
-            process = t_min <= tv_max + T_EPS;
-        }
-        if (process) {
-            if ((leaf >> local) & 1u) {
-                if (COUNT) cnt.leaves += 1;
-                const uint32_t below = (1u << local) - 1u;
-                const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
-                if (lidx < s.node_count) {
-                    const uint32_t brick = s.nodes[lidx].y & 0xFFFFFFu;
-                    if (brick != 0xFFFFFFu && brick < s.brick_count) {
-                        int bc[3];
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            const float lp = (mask & (1 << i)) ? pos[i] : (3.0f - scale_exp2 - pos[i]);
-                            bc[i] = (int)((lp - 1.0f) * s.Bf);
-                        }
-                        if (brick_dda<COUNT>(s, brick, bc, ol, dl, tstart, texit, h, cnt)) {
-                            h.flags |= (uint32_t)(iter + 1) << 16;
-                            h.leaf_node = lidx;
-                            if (COUNT) cnt.hits += 1;
-                            return true;
-                        }
+    for (int i = 0; i < 3; ++i)
+        if (__builtin_fabsf(du[i]) >= DIR_EPS) mask |= 8 << i;
+    r.idx = idx;
+    r.mask = mask;
+    r.t_min = t_min;
+    r.t_max = t_max;
+    return false;
+}
+
+__device__ __forceinline__ float corrected_tc_max_f(float tx, float ty, float tz, int mask, float t_max)
+{
+    const float thr = 1000.0f;
+    const float vx = ((mask & 8) && __builtin_fabsf(tx) < thr) ? tx : t_max;
+    const float vy = ((mask & 16) && __builtin_fabsf(ty) < thr) ? ty : t_max;
+    const float vz = ((mask & 32) && __builtin_fabsf(tz) < thr) ? tz : t_max;
+    return mn(mn(vx, vy), vz);
+}
+
+// One iteration of the ESVO loop (VoxelRayMarch_Compressed.comp:396-478).
+// Returns true when the ray is finished; h then holds its record.
+template <bool COUNT>
+__device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
+{
+    const int iter = r.iter;
+    if (!(iter < MAX_ITERS && r.scale <= ESVO_MAX_SCALE)) {
+        store_miss(h, (uint32_t)iter);
+        return true;
+    }
+    if (COUNT) cnt.iters += 1;
+    if (r.parent >= s.node_count) {
+        store_miss(h, (uint32_t)(iter + 1));
+        return true;
+    }
+    const uint2 desc = fetch_node(s, r.parent);
+    const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
+    const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
+    const int octm = r.mask & 7;
+    const int local = r.idx ^ ((~octm) & 7);
+
+    bool process = false;
+    float tv_max = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS)) {
+        const float tx = r.pos[0] * r.tc[0] - r.tb[0];
+        const float ty = r.pos[1] * r.tc[1] - r.tb[1];
+        const float tz = r.pos[2] * r.tc[2] - r.tb[2];
+        const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
+        tv_max = mn(r.t_max, tcm);
+        const float half = r.scale_exp2 * 0.5f;
+        cx = half * r.tc[0] + tx;
+        cy = half * r.tc[1] + ty;
+        cz = half * r.tc[2] + tz;
+        process = r.t_min <= tv_max + T_EPS;
+    }
+    if (process) {
+        if ((leaf >> local) & 1u) {
+            if (COUNT) cnt.leaves += 1;
+            const uint32_t below = (1u << local) - 1u;
+            const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
+            if (lidx < s.node_count) {
+                const uint32_t brick = fetch_node(s, lidx).y & 0xFFFFFFu;
+                if (brick != 0xFFFFFFu && brick < s.brick_count) {
+                    int bc[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
+                        bc[i] = (int)((lp - 1.0f) * s.Bf);
+                    }
+                    if (brick_dda<COUNT>(s, brick, bc, r.ol, r.dl, r.tstart, r.texit, h, cnt)) {
+                        h.flags |= (uint32_t)(iter + 1) << 16;
+                        h.leaf_node = lidx;
+                        if (COUNT) cnt.hits += 1;
+                        return true;
                     }
                 }
-                t_min = tv_max;
-            } else {
-                stk.put(ESVO_MAX_SCALE - scale, parent, t_max);
-                const uint32_t below = (1u << local) - 1u;
-                parent = cp + popc8(valid & ~leaf & below);
-                idx = 0;
-                scale -= 1;
-                const float half = scale_exp2 * 0.5f;
-                scale_exp2 = half;
-                if (cx > t_min) { idx ^= 1; pos[0] += scale_exp2; }
-                if (cy > t_min) { idx ^= 2; pos[1] += scale_exp2; }
-                if (cz > t_min) { idx ^= 4; pos[2] += scale_exp2; }
-                t_max = tv_max;
-                continue;
             }
-        }
-
-        // ADVANCE
-        const float tx = pos[0] * tc[0] - tb[0];
-        const float ty = pos[1] * tc[1] - tb[1];
-        const float tz = pos[2] * tc[2] - tb[2];
-        float tcm = corrected_tc_max(tx, ty, tz, du, t_max);
-        if (tcm >= 1e10f) tcm = mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f);
-        int step_mask = 0;
-        if (canx && tx <= tcm) { step_mask ^= 1; pos[0] -= scale_exp2; }
-        if (cany && ty <= tcm) { step_mask ^= 2; pos[1] -= scale_exp2; }
-        if (canz && tz <= tcm) { step_mask ^= 4; pos[2] -= scale_exp2; }
-        t_min = mx(tcm, 0.0f);
-        idx ^= step_mask;
-
-        if ((idx & step_mask) == 0) {
-            if (scale < ESVO_MAX_SCALE) t_max = stk.get_tmax(ESVO_MAX_SCALE - (scale + 1));
-            continue;
-        }
-
-        // POP
-        if (scale >= ESVO_MAX_SCALE) {
-            if (t_min > t_max || pos[0] < 1.0f || pos[0] >= 2.0f || pos[1] < 1.0f || pos[1] >= 2.0f ||
-                pos[2] < 1.0f || pos[2] >= 2.0f) {
-                store_miss(h, (uint32_t)(iter + 1));
-                return false;
-            }
-            continue;
-        }
-        uint32_t diff = 0u;
-        if (step_mask & 1) diff |= f2u(pos[0]) ^ f2u(pos[0] + scale_exp2);
-        if (step_mask & 2) diff |= f2u(pos[1]) ^ f2u(pos[1] + scale_exp2);
-        if (step_mask & 4) diff |= f2u(pos[2]) ^ f2u(pos[2] + scale_exp2);
-        if (diff == 0u) {
-            store_miss(h, (uint32_t)(iter + 1));
+            r.t_min = tv_max;
+        } else {
+            stack_put(s, ESVO_MAX_SCALE - r.scale, r.parent, r.t_max);
+            const uint32_t below = (1u << local) - 1u;
+            r.parent = cp + popc8(valid & ~leaf & below);
+            int idx = 0;
+            r.scale -= 1;
+            const float half = r.scale_exp2 * 0.5f;
+            r.scale_exp2 = half;
+            if (cx > r.t_min) { idx ^= 1; r.pos[0] += r.scale_exp2; }
+            if (cy > r.t_min) { idx ^= 2; r.pos[1] += r.scale_exp2; }
+            if (cz > r.t_min) { idx ^= 4; r.pos[2] += r.scale_exp2; }
+            r.idx = idx;
+            r.t_max = tv_max;
+            r.iter = iter + 1;
             return false;
         }
-        scale = (int)((f2u((float)diff) >> 23) - 127u);
-        scale_exp2 = u2f((uint32_t)(scale - ESVO_MAX_SCALE - 1 + 127) << 23);
-        if (scale < s.min_scale || scale > ESVO_MAX_SCALE) {
-            store_miss(h, (uint32_t)(iter + 1));
-            return false;
-        }
-        parent = stk.get_ptr(ESVO_MAX_SCALE - scale);
-        t_max = stk.get_tmax(ESVO_MAX_SCALE - scale);
-        const uint32_t shx = f2u(pos[0]) >> scale, shy = f2u(pos[1]) >> scale, shz = f2u(pos[2]) >> scale;
-        pos[0] = u2f(shx << scale);
-        pos[1] = u2f(shy << scale);
-        pos[2] = u2f(shz << scale);
-        idx = (int)((shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2));
     }
-    store_miss(h, (uint32_t)iter);
+
+    // ADVANCE
+    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
+    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
+    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
+    const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
+    float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
+    if (tcm >= 1e10f) tcm = mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f);
+    int step_mask = 0;
+    if (canx && tx <= tcm) { step_mask ^= 1; r.pos[0] -= r.scale_exp2; }
+    if (cany && ty <= tcm) { step_mask ^= 2; r.pos[1] -= r.scale_exp2; }
+    if (canz && tz <= tcm) { step_mask ^= 4; r.pos[2] -= r.scale_exp2; }
+    r.t_min = mx(tcm, 0.0f);
+    r.idx ^= step_mask;
+    r.iter = iter + 1;
+
+    if ((r.idx & step_mask) == 0) {
+        if (r.scale < ESVO_MAX_SCALE) r.t_max = stack_tmax(s, ESVO_MAX_SCALE - (r.scale + 1));
+        return false;
+    }
+
+    // POP
+    if (r.scale >= ESVO_MAX_SCALE) {
+        if (r.t_min > r.t_max || r.pos[0] < 1.0f || r.pos[0] >= 2.0f || r.pos[1] < 1.0f || r.pos[1] >= 2.0f ||
+            r.pos[2] < 1.0f || r.pos[2] >= 2.0f) {
+            store_miss(h, (uint32_t)(iter + 1));
+            return true;
+        }
+        return false;
+    }
+    uint32_t diff = 0u;
+    if (step_mask & 1) diff |= f2u(r.pos[0]) ^ f2u(r.pos[0] + r.scale_exp2);
+    if (step_mask & 2) diff |= f2u(r.pos[1]) ^ f2u(r.pos[1] + r.scale_exp2);
+    if (step_mask & 4) diff |= f2u(r.pos[2]) ^ f2u(r.pos[2] + r.scale_exp2);
+    if (diff == 0u) {
+        store_miss(h, (uint32_t)(iter + 1));
+        return true;
+    }
+    const int scale = (int)((f2u((float)diff) >> 23) - 127u);
+    r.scale = scale;
+    r.scale_exp2 = u2f((uint32_t)(scale - ESVO_MAX_SCALE - 1 + 127) << 23);
+    if (scale < s.min_scale || scale > ESVO_MAX_SCALE) {
+        store_miss(h, (uint32_t)(iter + 1));
+        return true;
+    }
+    r.parent = stack_ptr(s, ESVO_MAX_SCALE - scale);
+    r.t_max = stack_tmax(s, ESVO_MAX_SCALE - scale);
+    const uint32_t shx = f2u(r.pos[0]) >> scale, shy = f2u(r.pos[1]) >> scale, shz = f2u(r.pos[2]) >> scale;
+    r.pos[0] = u2f(shx << scale);
+    r.pos[1] = u2f(shy << scale);
+    r.pos[2] = u2f(shz << scale);
+    r.idx = (int)((shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2));
     return false;
+}
+
+// Full traversal of one ray. Writes the hit record; returns true on a hit.
+template <bool COUNT>
+__device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
+                                          float tmax, vx_hit& h, TraceCounts& cnt)
+{
+    RayState r;
+    if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
+    while (!step_ray<COUNT>(s, r, h, cnt)) {
+    }
+    return (h.flags & 1u) != 0u;
 }
 
 } // namespace vx
