@@ -242,6 +242,35 @@ vx_status vx_batch_graph_create(vx_batch* batch, const vx_ray* rays, const uint6
 vx_status vx_graph_launch(vx_graph* graph, vx_stream stream);
 vx_status vx_graph_destroy(vx_graph* graph);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiable albedo rendering (no reference counterpart; the shading    */
+/* is Lambertian as shaders/Lighting.glsl:19-28 without the ambient term)    */
+/* ------------------------------------------------------------------------ */
+
+#define VX_GRAD_ATOMIC 0u
+#define VX_GRAD_DETERMINISTIC 1u
+
+/* Occupied voxels of the scene = rows of its albedo array (float[rows][3]).
+ * The row of a voxel is the number of occupied voxels before it in brick-id,
+ * then linear (x + 8y + 64z) order. */
+vx_status vx_scene_voxel_count(vx_scene* scene, uint64_t* n_voxels);
+
+/* image[p] = albedo[row(hit voxel of pixel p)] * max(dot(n, light), 0), 0 on a
+ * miss, for the pinhole rays of vx_render (n = entry-face normal, light a unit
+ * vector). image is float[h*w][3]. When hit_row/hit_shade (u32/float[h*w]) are
+ * given they receive each pixel's row (~0u on a miss) and shade. */
+vx_status vx_albedo_forward(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
+                            const float light[3], const float* albedo, float* image, uint32_t* hit_row,
+                            float* hit_shade, vx_stream stream);
+
+/* Adjoint: grad_albedo[row] += grad_image[p] * shade[p] over the pixels p that
+ * hit the row. hit_row/hit_shade from vx_albedo_forward, or NULL to re-trace the
+ * hits. mode VX_GRAD_ATOMIC (FP32 atomics) or VX_GRAD_DETERMINISTIC (stable sort
+ * by row + reduce by key: identical bits on every run). grad_albedo accumulates. */
+vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
+                             const float light[3], const float* grad_image, const uint32_t* hit_row,
+                             const float* hit_shade, float* grad_albedo, uint32_t mode, vx_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

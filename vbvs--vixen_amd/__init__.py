@@ -138,6 +138,9 @@ def _load() -> C.CDLL:
         "vx_batch_graph_create": ([vp, vp, C.POINTER(u64), vp, C.POINTER(VxTuning), C.POINTER(vp)], st),
         "vx_graph_launch": ([vp, vp], st),
         "vx_graph_destroy": ([vp], st),
+        "vx_scene_voxel_count": ([vp, C.POINTER(u64)], st),
+        "vx_albedo_forward": ([vp, C.POINTER(VxCamera), u32, u32, vp, vp, vp, vp, vp, vp], st),
+        "vx_albedo_backward": ([vp, C.POINTER(VxCamera), u32, u32, vp, vp, vp, vp, vp, u32, vp], st),
         "vx_gen_cornell": ([u32, u64, vp, C.POINTER(VxCamera)], st),
         "vx_gen_cave": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
         "vx_gen_urban": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
@@ -157,7 +160,7 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
-            "vx_graph_destroy")
+            "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward")
 
 
 def _check(status: int, where: str) -> None:
@@ -197,6 +200,38 @@ class BuiltScene:
     def device_bytes(self) -> int:
         nb = self.brick_count
         return self.node_count * 8 + nb * 2048 + nb * 64 + (nb * 256 if self.bc1 is not None else 0)
+
+
+def make_camera(pos, look_dir, fov_deg: float, aspect: float) -> VxCamera:
+    """Pinhole camera: right = dir x worldUp (y up, z when looking straight up/down), up = right x dir."""
+    d = np.asarray(look_dir, np.float64)
+    d = d / np.linalg.norm(d)
+    wu = np.array([0.0, 0.0, 1.0]) if abs(d[1]) > 0.99 else np.array([0.0, 1.0, 0.0])
+    r = np.cross(d, wu)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, d)
+    cam = VxCamera()
+    cam.pos[:] = [float(x) for x in np.asarray(pos, np.float64)]
+    cam.dir[:], cam.up[:], cam.right[:] = [float(x) for x in d], [float(x) for x in u], [float(x) for x in r]
+    cam.fov_deg, cam.aspect = float(fov_deg), float(aspect)
+    return cam
+
+
+def seeded_views(grid: np.ndarray, seed: int, n: int, fov_deg: float = 70.0, aspect: float = 16 / 9,
+                 clearance: int = 2) -> list[VxCamera]:
+    """n cameras at seeded empty cells (a (2c+1)^3 empty neighbourhood) with seeded look directions."""
+    rng = np.random.default_rng(seed)
+    N = grid.shape[0]
+    solid = grid != 0
+    cams = []
+    while len(cams) < n:
+        x, y, z = rng.integers(clearance, N - clearance, 3)
+        c = clearance
+        if solid[z - c:z + c + 1, y - c:y + c + 1, x - c:x + c + 1].any():
+            continue
+        v = rng.normal(size=3)
+        cams.append(make_camera((x + 0.5, y + 0.5, z + 0.5), v, fov_deg, aspect))
+    return cams
 
 
 def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0) -> BuiltScene:
@@ -348,6 +383,39 @@ class Scene:
     def set_lds_depth(self, depth: int) -> None:
         _check(LIB.vx_scene_set_lds_depth(self._h, depth), "vx_scene_set_lds_depth")
 
+    # ---- differentiable albedo rendering (Lambertian, one RGB albedo per occupied voxel) ----
+    def voxel_count(self) -> int:
+        n = C.c_uint64()
+        _check(LIB.vx_scene_voxel_count(self._h, C.byref(n)), "vx_scene_voxel_count")
+        return int(n.value)
+
+    def albedo_forward(self, cam: VxCamera, w: int, h: int, light, albedo, store_hits: bool = True, stream=None):
+        """image [h,w,3] (+ per-pixel albedo row and shade when store_hits)."""
+        torch = _torch()
+        assert albedo.dtype == torch.float32 and albedo.is_contiguous() and albedo.shape == (self.voxel_count(), 3)
+        lt = np.ascontiguousarray(light, dtype=np.float32)
+        image = torch.empty((h, w, 3), dtype=torch.float32, device=albedo.device)
+        rows = torch.empty(h * w, dtype=torch.int32, device=albedo.device) if store_hits else None
+        shade = torch.empty(h * w, dtype=torch.float32, device=albedo.device) if store_hits else None
+        _check(LIB.vx_albedo_forward(self._h, C.byref(cam), w, h, lt.ctypes.data, albedo.data_ptr(), image.data_ptr(),
+                                     rows.data_ptr() if store_hits else None, shade.data_ptr() if store_hits else None,
+                                     _stream_ptr(stream)), "vx_albedo_forward")
+        return image, rows, shade
+
+    def albedo_backward(self, cam: VxCamera, w: int, h: int, light, grad_image, rows=None, shade=None, grad=None,
+                        deterministic: bool = True, stream=None):
+        """grad [V,3] += adjoint of albedo_forward for grad_image [h,w,3] (hits re-traced when rows is None)."""
+        torch = _torch()
+        assert grad_image.dtype == torch.float32 and grad_image.is_contiguous() and grad_image.numel() == h * w * 3
+        if grad is None:
+            grad = torch.zeros((self.voxel_count(), 3), dtype=torch.float32, device=grad_image.device)
+        lt = np.ascontiguousarray(light, dtype=np.float32)
+        _check(LIB.vx_albedo_backward(self._h, C.byref(cam), w, h, lt.ctypes.data, grad_image.data_ptr(),
+                                      rows.data_ptr() if rows is not None else None,
+                                      shade.data_ptr() if shade is not None else None, grad.data_ptr(),
+                                      1 if deterministic else 0, _stream_ptr(stream)), "vx_albedo_backward")
+        return grad
+
 
 def _tuning(t) -> VxTuning | None:
     if t is None or isinstance(t, VxTuning):
@@ -426,6 +494,37 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+_TORCH_EXT = None
+
+
+def torch_ext():
+    """The libtorch autograd binding (vx_torch.so): albedo_render with a hand-written backward."""
+    global _TORCH_EXT
+    if _TORCH_EXT is None:
+        import importlib.machinery
+        import importlib.util
+        path = PKG_DIR / "vx_torch.so"
+        if not path.exists():
+            raise ImportError(f"{path} is not built: run __graft_entry__.build()")
+        loader = importlib.machinery.ExtensionFileLoader("vx_torch", str(path))
+        spec = importlib.util.spec_from_file_location("vx_torch", str(path), loader=loader)
+        mod = importlib.util.module_from_spec(spec)
+        loader.exec_module(mod)
+        _TORCH_EXT = mod
+    return _TORCH_EXT
+
+
+def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, store_hits: bool = True,
+                  deterministic: bool = True, stream=None):
+    """Differentiable render (torch::autograd::Function in vx_torch.so): image [h,w,3]; albedo gets gradients."""
+    torch = _torch()
+    camt = torch.from_numpy(cam.as_floats())
+    lt = torch.tensor(np.asarray(light, dtype=np.float32))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return torch_ext().albedo_render(albedo, int(scene._h.value), camt, int(w), int(h), lt, bool(store_hits),
+                                     bool(deterministic), int(s.cuda_stream))
 
 
 def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None):

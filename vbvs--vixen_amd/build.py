@@ -39,7 +39,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp"]
-HIP_SOURCES = ["vx_api.hip", "vx_batch.hip"]
+HIP_SOURCES = ["vx_api.hip", "vx_batch.hip", "vx_albedo.hip"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -79,10 +79,40 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+TORCH_EXT = PKG / "vx_torch.so"
+
+
+def build_torch_ext(verbose: bool = False) -> Path | None:
+    """The libtorch autograd binding (vx_torch.so), linked against libvixen_hip.so."""
+    try:
+        import torch
+        import pybind11
+    except ImportError:
+        return None
+    import sysconfig
+    tdir = Path(torch.__file__).resolve().parent
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [CXX, "-O2", "-fPIC", "-shared", "-std=c++17", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-DTORCH_EXTENSION_NAME=vx_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", "-w",
+           f"-I{tdir / 'include'}", f"-I{tdir / 'include' / 'torch' / 'csrc' / 'api' / 'include'}",
+           f"-I{sysconfig.get_paths()['include']}", f"-I{pybind11.get_include()}", f"-I{INCLUDE}",
+           str(CSRC / "vx_torch.cpp"), "-o", str(TORCH_EXT.with_suffix(".so.tmp")),
+           f"-L{tdir / 'lib'}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+           f"-L{PKG}", "-l:libvixen_hip.so", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{tdir / 'lib'}"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(TORCH_EXT.with_suffix(".so.tmp"), TORCH_EXT)
+    return TORCH_EXT
+
+
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
     LIB.unlink(missing_ok=True)
+    TORCH_EXT.unlink(missing_ok=True)
 
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    if "--no-torch" not in sys.argv:
+        print(build_torch_ext(verbose="-v" in sys.argv))

@@ -1,0 +1,401 @@
+// vx_albedo.hip -- differentiable Lambertian albedo rendering and its adjoint.
+//
+// No reference counterpart: VIXEN renders only. The forward pass reuses the
+// traversal of vx_render unchanged (the discrete hit selection is not
+// differentiated); per pixel it shades a per-voxel FP32 RGB albedo,
+//   pixel = albedo[v] * max(dot(n, light), 0),
+// with n the entry-face normal of the hit voxel (the axis the DDA crossed
+// into it) and light a fixed unit vector; misses are 0. Parameters are one
+// row per occupied voxel: row(v) = base[brick] + (occupied voxels before v in
+// the brick's linear order), read from the occupancy mask.
+//
+// The adjoint pushes dL/dpixel through the shading into dL/dalbedo:
+//   dL/dalbedo[v] += dL/dpixel[p] * shade[p]   for every pixel p that hit v.
+// Hits are either read back from the forward pass (8 bytes per pixel) or
+// recomputed by re-tracing (no per-pixel storage). Two reductions:
+//   atomic         one FP32 atomic add per hit pixel and channel;
+//   deterministic  (row, contribution) pairs are radix-sorted by row (stable,
+//                  so equal rows keep pixel order), reduced by key, and the
+//                  per-row sums added without conflicts: identical bits on
+//                  every run.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+
+#include "vx_internal.hpp"
+
+namespace vx {
+
+constexpr uint32_t kAlbBlock = 256;
+constexpr uint32_t kNoVoxel = 0xFFFFFFFFu;
+
+struct Rgb {
+    float r, g, b;
+};
+struct RgbSum {
+    __host__ __device__ Rgb operator()(const Rgb& a, const Rgb& b) const { return Rgb{a.r + b.r, a.g + b.g, a.b + b.b}; }
+};
+
+struct AlbCam {
+    float pos[3], dir[3], up[3], right[3];
+    float tan_half, aspect;
+    float light[3];
+};
+
+__global__ void __launch_bounds__(kAlbBlock) k_brick_popc(const uint32_t* __restrict__ occ, uint64_t nb,
+                                                         uint32_t* __restrict__ counts)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * kAlbBlock + threadIdx.x;
+    if (b >= nb) return;
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) c += __builtin_popcount(occ[b * 16 + w]);
+    counts[b] = c;
+}
+
+// Row of the hit voxel: brick base plus occupied voxels before `lin` in the brick.
+__device__ __forceinline__ uint32_t voxel_row(const SceneView& s, const uint32_t* __restrict__ base, uint32_t brick,
+                                              uint32_t lin)
+{
+    const uint32_t* m = s.occ + (size_t)brick * 16u;
+    const uint32_t wq = lin >> 5;
+    uint32_t r = base[brick];
+    for (uint32_t w = 0; w < wq; ++w) r += __builtin_popcount(m[w]);
+    return r + __builtin_popcount(m[wq] & ((1u << (lin & 31u)) - 1u));
+}
+
+__device__ __forceinline__ float face_shade(uint32_t flags, const float light[3])
+{
+    const uint32_t axis = (flags >> 1) & 3u;
+    const float l0 = light[0], l1 = light[1], l2 = light[2];
+    const float l = axis == 0u ? l0 : (axis == 1u ? l1 : l2);
+    const float ndl = (flags & 8u) ? l : -l;
+    return ndl > 0.0f ? ndl : 0.0f;
+}
+
+// Traces pixel p of a w x h image; returns its albedo row (kNoVoxel on a miss) and shade.
+__device__ __forceinline__ uint32_t trace_pixel(const SceneView& s, const AlbCam& c, const uint32_t* base,
+                                                uint32_t px, uint32_t py, uint32_t w, uint32_t h, float& shade)
+{
+    const float u = ((float)px + 0.5f) / (float)w;
+    const float v = ((float)py + 0.5f) / (float)h;
+    const float ndcx = u * 2.0f - 1.0f;
+    const float ndcy = -(v * 2.0f - 1.0f);
+    float d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a = ((c.right[i] * ndcx) * c.tan_half) * c.aspect;
+        const float b = (c.up[i] * ndcy) * c.tan_half;
+        d[i] = (c.dir[i] + a) + b;
+    }
+    vx_hit hr;
+    TraceCounts cnt{0u, 0u, 0u, 0u};
+    shade = 0.0f;
+    if (!trace_ray<false>(s, c.pos, d, 0.0f, 3.0e38f, hr, cnt)) return kNoVoxel;
+    shade = face_shade(hr.flags, c.light);
+    return voxel_row(s, base, hr.brick, hr.voxel);
+}
+
+__device__ __forceinline__ void stage_alb(const SceneView& s)
+{
+    for (uint32_t i = threadIdx.x; i < s.lds_count; i += blockDim.x) vx_lds_nodes[i] = s.nodes[i];
+    __syncthreads();
+}
+
+// Pixel of lane threadIdx.x in 16x16 tile blockIdx.x (row-major tiles, 8x8 per wave).
+__device__ __forceinline__ bool tile_pixel(uint32_t w, uint32_t h, uint32_t& px, uint32_t& py)
+{
+    const uint32_t tiles_x = (w + 15u) / 16u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    px = (blockIdx.x % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    py = (blockIdx.x / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    return px < w && py < h;
+}
+
+__global__ void __launch_bounds__(kAlbBlock)
+    k_albedo_forward(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
+                     const float* __restrict__ albedo, float* __restrict__ image, uint32_t* __restrict__ hit_row,
+                     float* __restrict__ hit_shade)
+{
+    stage_alb(s);
+    uint32_t px, py;
+    if (!tile_pixel(w, h, px, py)) return;
+    float shade;
+    const uint32_t row = trace_pixel(s, c, base, px, py, w, h, shade);
+    const uint64_t p = (uint64_t)py * w + px;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    if (row != kNoVoxel) {
+        r = albedo[(size_t)row * 3 + 0] * shade;
+        g = albedo[(size_t)row * 3 + 1] * shade;
+        b = albedo[(size_t)row * 3 + 2] * shade;
+    }
+    image[p * 3 + 0] = r;
+    image[p * 3 + 1] = g;
+    image[p * 3 + 2] = b;
+    if (hit_row) {
+        hit_row[p] = row;
+        hit_shade[p] = shade;
+    }
+}
+
+// Per-pixel contribution dL/dpixel * shade. ATOMIC adds it straight into grad;
+// otherwise it emits a (row, contribution) pair for the sorted reduction.
+template <bool RECOMPUTE, bool ATOMIC>
+__global__ void __launch_bounds__(kAlbBlock)
+    k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
+                     const float* __restrict__ grad_image, const uint32_t* __restrict__ hit_row,
+                     const float* __restrict__ hit_shade, float* __restrict__ grad, uint32_t* __restrict__ keys,
+                     Rgb* __restrict__ vals)
+{
+    if (RECOMPUTE) stage_alb(s);
+    uint32_t px, py;
+    if (!tile_pixel(w, h, px, py)) return;
+    const uint64_t p = (uint64_t)py * w + px;
+    uint32_t row;
+    float shade;
+    if (RECOMPUTE) {
+        row = trace_pixel(s, c, base, px, py, w, h, shade);
+    } else {
+        row = hit_row[p];
+        shade = hit_shade[p];
+    }
+    const float gr = grad_image[p * 3 + 0] * shade, gg = grad_image[p * 3 + 1] * shade,
+                gb = grad_image[p * 3 + 2] * shade;
+    if (ATOMIC) {
+        if (row != kNoVoxel) {
+            atomicAdd(&grad[(size_t)row * 3 + 0], gr);
+            atomicAdd(&grad[(size_t)row * 3 + 1], gg);
+            atomicAdd(&grad[(size_t)row * 3 + 2], gb);
+        }
+    } else {
+        keys[p] = row;
+        vals[p] = Rgb{gr, gg, gb};
+    }
+}
+
+__global__ void __launch_bounds__(kAlbBlock) k_scatter_runs(const uint32_t* __restrict__ rows,
+                                                           const Rgb* __restrict__ sums,
+                                                           const uint32_t* __restrict__ n_runs,
+                                                           float* __restrict__ grad)
+{
+    const uint32_t i = blockIdx.x * kAlbBlock + threadIdx.x;
+    if (i >= *n_runs) return;
+    const uint32_t row = rows[i];
+    if (row == kNoVoxel) return;
+    grad[(size_t)row * 3 + 0] += sums[i].r;
+    grad[(size_t)row * 3 + 1] += sums[i].g;
+    grad[(size_t)row * 3 + 2] += sums[i].b;
+}
+
+} // namespace vx
+
+// Per-scene state of the albedo path (built on first use).
+struct vx_albedo_state {
+    uint32_t* base = nullptr; // per-brick row base
+    uint64_t voxels = 0;
+    // sorted-reduction workspace, grown on demand
+    size_t cap_px = 0, cap_tmp = 0;
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *rows = nullptr, *n_runs = nullptr;
+    vx::Rgb *vals = nullptr, *vals_sorted = nullptr, *sums = nullptr;
+    void* tmp = nullptr;
+};
+
+namespace vx {
+
+static vx_albedo_state* albedo_state(vx_scene* s, hipStream_t st, vx_status& status)
+{
+    status = VX_OK;
+    if (s->albedo) return s->albedo;
+    auto* a = new (std::nothrow) vx_albedo_state{};
+    if (!a) {
+        status = fail(VX_ERR_OUT_OF_MEMORY, "albedo state: out of memory");
+        return nullptr;
+    }
+    const uint64_t nb = s->brick_count;
+    uint32_t* counts = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint32_t last[2] = {0, 0};
+    bool ok = hipMalloc(&a->base, std::max<uint64_t>(1, nb) * 4) == hipSuccess &&
+              hipMalloc(&counts, std::max<uint64_t>(1, nb) * 4) == hipSuccess;
+    if (ok && nb) {
+        k_brick_popc<<<dim3((unsigned)((nb + kAlbBlock - 1) / kAlbBlock)), dim3(kAlbBlock), 0, st>>>(s->occ, nb, counts);
+        ok = hipGetLastError() == hipSuccess &&
+             hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, a->base, (int)nb, st) == hipSuccess &&
+             hipMalloc(&tmp, std::max<size_t>(1, tmp_bytes)) == hipSuccess &&
+             hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, a->base, (int)nb, st) == hipSuccess &&
+             hipMemcpyAsync(&last[0], a->base + nb - 1, 4, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipMemcpyAsync(&last[1], counts + nb - 1, 4, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+    }
+    (void)hipFree(counts);
+    (void)hipFree(tmp);
+    if (!ok) {
+        (void)hipFree(a->base);
+        delete a;
+        status = fail(VX_ERR_HIP, "albedo state: voxel row table failed");
+        return nullptr;
+    }
+    a->voxels = nb ? (uint64_t)last[0] + last[1] : 0;
+    s->albedo = a;
+    return a;
+}
+
+static AlbCam make_alb_cam(const vx_camera* cam, const float light[3])
+{
+    AlbCam c;
+    for (int i = 0; i < 3; ++i) {
+        c.pos[i] = cam->pos[i];
+        c.dir[i] = cam->dir[i];
+        c.up[i] = cam->up[i];
+        c.right[i] = cam->right[i];
+        c.light[i] = light[i];
+    }
+    c.tan_half = std::tan((cam->fov_deg * 0.5f) * 0.017453292519943295f);
+    c.aspect = cam->aspect;
+    return c;
+}
+
+static SceneView alb_view(const vx_scene* s, uint32_t& lds)
+{
+    SceneView v = s->view;
+    v.lds_off = 0;
+    v.lds_count = staged_nodes(s, s->lds_depth);
+    const uint32_t cap = (kMaxLdsBytes - stack_bytes(kAlbBlock)) / 8u;
+    if (v.lds_count > cap) v.lds_count = cap;
+    v.stack_base = v.lds_count * 2u;
+    lds = v.lds_count * 8u + stack_bytes(kAlbBlock);
+    return v;
+}
+
+static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
+{
+    if (px > a->cap_px) {
+        for (void* p : {(void*)a->keys, (void*)a->keys_sorted, (void*)a->rows, (void*)a->n_runs, (void*)a->vals,
+                        (void*)a->vals_sorted, (void*)a->sums})
+            (void)hipFree(p);
+        a->cap_px = 0;
+        if (hipMalloc(&a->keys, px * 4) != hipSuccess || hipMalloc(&a->keys_sorted, px * 4) != hipSuccess ||
+            hipMalloc(&a->rows, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 4) != hipSuccess ||
+            hipMalloc(&a->vals, px * sizeof(Rgb)) != hipSuccess ||
+            hipMalloc(&a->vals_sorted, px * sizeof(Rgb)) != hipSuccess ||
+            hipMalloc(&a->sums, px * sizeof(Rgb)) != hipSuccess)
+            return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
+        a->cap_px = px;
+    }
+    if (tmp > a->cap_tmp) {
+        (void)hipFree(a->tmp);
+        a->cap_tmp = 0;
+        if (hipMalloc(&a->tmp, tmp) != hipSuccess) return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
+        a->cap_tmp = tmp;
+    }
+    return VX_OK;
+}
+
+} // namespace vx
+
+void vx_albedo_state_free(vx_albedo_state* a)
+{
+    if (!a) return;
+    for (void* p : {(void*)a->base, (void*)a->keys, (void*)a->keys_sorted, (void*)a->rows, (void*)a->n_runs,
+                    (void*)a->vals, (void*)a->vals_sorted, (void*)a->sums, a->tmp})
+        (void)hipFree(p);
+    delete a;
+}
+
+extern "C" {
+
+vx_status vx_scene_voxel_count(vx_scene* s, uint64_t* n_voxels)
+{
+    if (!s || !n_voxels) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_voxel_count: null argument");
+    vx_status st;
+    vx_albedo_state* a = vx::albedo_state(s, nullptr, st);
+    if (!a) return st;
+    *n_voxels = a->voxels;
+    return VX_OK;
+}
+
+vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
+                            const float* albedo, float* image, uint32_t* hit_row, float* hit_shade, vx_stream stream)
+{
+    if (!s || !cam || !light || !albedo || !image || (!hit_row) != (!hit_shade))
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: bad arguments");
+    if (w == 0 || h == 0) return VX_OK;
+    if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: image too large");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    vx_status status;
+    vx_albedo_state* a = vx::albedo_state(s, st, status);
+    if (!a) return status;
+    uint32_t lds = 0;
+    const vx::SceneView v = vx::alb_view(s, lds);
+    const vx::AlbCam c = vx::make_alb_cam(cam, light);
+    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+    vx::k_albedo_forward<<<dim3(tiles), dim3(vx::kAlbBlock), lds, st>>>(v, c, w, h, a->base, albedo, image, hit_row,
+                                                                        hit_shade);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_forward: launch failed");
+    return VX_OK;
+}
+
+vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
+                             const float* grad_image, const uint32_t* hit_row, const float* hit_shade,
+                             float* grad_albedo, uint32_t mode, vx_stream stream)
+{
+    if (!s || !cam || !light || !grad_image || !grad_albedo || (!hit_row) != (!hit_shade) || mode > 1)
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: bad arguments");
+    if (w == 0 || h == 0) return VX_OK;
+    if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: image too large");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    vx_status status;
+    vx_albedo_state* a = vx::albedo_state(s, st, status);
+    if (!a) return status;
+    uint32_t lds = 0;
+    const vx::SceneView v = vx::alb_view(s, lds);
+    const vx::AlbCam c = vx::make_alb_cam(cam, light);
+    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+    const bool recompute = hit_row == nullptr;
+    const dim3 grid(tiles), block(vx::kAlbBlock);
+    const size_t px = (size_t)w * h;
+    if (mode == VX_GRAD_ATOMIC) {
+        if (recompute)
+            vx::k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr,
+                                                                       nullptr, grad_albedo, nullptr, nullptr);
+        else
+            vx::k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
+                                                                      hit_shade, grad_albedo, nullptr, nullptr);
+        if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
+        return VX_OK;
+    }
+    // deterministic: sort (row, contribution) by row, reduce by key, scatter the run sums
+    const int end_bit = 32; // misses carry the all-ones key and sort last
+    size_t sort_bytes = 0, red_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (vx::Rgb*)nullptr, (vx::Rgb*)nullptr, (int)px, 0, end_bit, st) != hipSuccess ||
+        hipcub::DeviceReduce::ReduceByKey(nullptr, red_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (vx::Rgb*)nullptr,
+                                          (vx::Rgb*)nullptr, (uint32_t*)nullptr, vx::RgbSum(), (int)px, st) != hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: workspace query failed");
+    if (vx_status g = vx::grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
+    if (recompute)
+        vx::k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
+                                                                    nullptr, a->keys, a->vals);
+    else
+        vx::k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
+                                                                   hit_shade, nullptr, a->keys, a->vals);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
+    size_t tb = a->cap_tmp;
+    if (hipcub::DeviceRadixSort::SortPairs(a->tmp, tb, a->keys, a->keys_sorted, a->vals, a->vals_sorted, (int)px, 0,
+                                           end_bit, st) != hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: sort failed");
+    tb = a->cap_tmp;
+    if (hipcub::DeviceReduce::ReduceByKey(a->tmp, tb, a->keys_sorted, a->rows, a->vals_sorted, a->sums, a->n_runs,
+                                          vx::RgbSum(), (int)px, st) != hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: reduce failed");
+    vx::k_scatter_runs<<<dim3((unsigned)((px + vx::kAlbBlock - 1) / vx::kAlbBlock)), block, 0, st>>>(
+        a->rows, a->sums, a->n_runs, grad_albedo);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: scatter failed");
+    return VX_OK;
+}
+
+} // extern "C"

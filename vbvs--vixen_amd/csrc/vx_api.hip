@@ -438,6 +438,7 @@ vx_status vx_scene_validate(const vx_scene_desc* desc)
 vx_status vx_scene_destroy(vx_scene* s)
 {
     if (!s) return VX_OK;
+    vx_albedo_state_free(s->albedo);
     (void)hipFree(s->nodes);
     (void)hipFree(s->brick_data);
     (void)hipFree(s->occ);

@@ -11,6 +11,9 @@
 #include "vx_common.hpp"
 #include "vx_trace.hpp"
 
+struct vx_albedo_state;
+void vx_albedo_state_free(vx_albedo_state* a);
+
 struct vx_ctx {
     int device;
     int cu_count;
@@ -30,6 +33,7 @@ struct vx_scene {
     // the descriptor array when it is in breadth-first order)
     uint32_t level_prefix[4];
     uint32_t lds_depth; // levels staged in LDS by vx_trace / vx_render (0 = none)
+    vx_albedo_state* albedo = nullptr; // differentiable-rendering state, built on first use
 };
 
 namespace vx {
