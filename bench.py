@@ -1,26 +1,36 @@
-"""Benchmark: primary-ray throughput on the 256^3 BC1 cave at 1920x1080 on one MI355X.
+"""Benchmark on one MI355X: the voxel ray-cast hot path (ESVO -> brick DDA -> BC1).
 
-A step is one vx_render launch: 2,073,600 pinhole rays traced through the ESVO
+Headline (value): primary-ray throughput on the 256^3 BC1 cave at 1920x1080. A
+step is one vx_render launch: 2,073,600 pinhole rays traced through the ESVO
 octree, brick DDA and BC1 decode, writing one 32-byte hit record per ray. The
 scene is built and uploaded before timing; the timed region holds exactly
 --steps launches bracketed by synchronisation, measured with HIP events on the
 stream the kernels are launched on.
 
-Prints ONE JSON line. Besides the required keys it carries:
+Two more legs ride in the same JSON line:
+  batched   64 seeded scenes (cave/urban, edges drawn from {64,128,256,512},
+            occupancy 10-90%, BC1), ray counts log-uniform in [1e4, 2e6], half
+            pinhole and half random-direction rays: one persistent batched launch
+            (autotuned, and its hipGraph replay) against 64 per-scene vx_trace
+            launches on the same stream; outputs compared byte for byte.
+  fwd_bwd   differentiable albedo on the 256^3 cave, 4 seeded 1280x720 views:
+            forward (render + per-voxel albedo shading) and the adjoint, timed
+            per view, and their ratio.
+Also:
   roofline      algorithmic bytes per launch / average launch time against the
-                8 TB/s HBM3E peak; the per-ray byte model is
-                8*iterations + 8*brick visits + 4*DDA voxels + 8*hits + 32 (record out),
-                counted by the kernel's own counting variant on the same frame;
-                traffic is the measured HBM bytes per launch when a rocprofv3 PMC
-                summary for this workload is present under profiles/
-  cpu_baseline  the strict-FP32 C oracle on the host cores, same frame, all cores
-                (median of 5 after a warm-up) -- a reported baseline, not a target
-  parity        the last timed frame compared bit for bit with the oracle's frame
+                8 TB/s HBM3E peak; per ray: 8*iterations + 8*brick visits +
+                4*DDA voxels + 8*hits + 32 (record out), counted by the kernel's
+                own counting variant on the same frame; traffic = measured HBM
+                bytes per launch from the committed rocprofv3 PMC summary
+  cpu_baseline  the strict-FP32 C oracle on the host cores, same frame, all
+                cores (median of 5 after a warm-up): reported, not a target
+  parity        the last timed frame compared bit for bit with the oracle's
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -49,6 +59,20 @@ def measured_traffic() -> float | None:
     return float(m.group(1)) if m else None
 
 
+def timed(torch, stream, fn, steps: int, warmup: int) -> float:
+    """Average milliseconds of fn() over `steps` calls (HIP events on `stream`)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def cpu_baseline(orc, built, rays, reps: int = 5):
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 64))
@@ -58,11 +82,151 @@ def cpu_baseline(orc, built, rays, reps: int = 5):
     out = None
     for _ in range(reps):
         t0 = time.perf_counter()
-        out, counts = orc.trace(scene, rays, nthreads=threads)
+        out, _ = orc.trace(scene, rays, nthreads=threads)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     return {"value": len(rays) / med / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"the full {len(rays)}-ray frame, {threads} threads, median of {reps} after a warm-up"}, out
+
+
+# ---------------------------------------------------------------------------
+# batched leg
+# ---------------------------------------------------------------------------
+def batched_leg(vx, torch, ctx, stream, steps: int):
+    rng = np.random.default_rng(4242)
+    scenes, offsets, dev_rays = [], [0], []
+    t0 = time.perf_counter()
+    total = 0
+    edges = []
+    for s in range(64):
+        N = int(rng.choice([64, 128, 256, 512]))
+        kind = "cave" if rng.random() < 0.5 else "urban"
+        occ = float(rng.uniform(0.1, 0.9))
+        n = int(math.exp(rng.uniform(math.log(1e4), math.log(2e6))))
+        grid, cam = (vx.gen_cave if kind == "cave" else vx.gen_urban)(N, 1000 + s, occ)
+        built = vx.build_from_dense(grid, bc1=True)
+        del grid
+        scenes.append(vx.Scene(ctx, built))
+        edges.append(N)
+        # half pinhole rays from the scene camera, half random-direction rays from inside the volume
+        n_cam = n // 2
+        w = max(1, int(math.ceil(math.sqrt(n_cam * 16 / 9))))
+        h = max(1, (n_cam + w - 1) // w)
+        cam_rays = vx.camera_rays(cam, w, h, 3.0e38).view(torch.float32).view(-1, 8)[:n_cam]
+        m = n - n_cam
+        g = torch.Generator(device="cuda").manual_seed(77 + s)
+        rnd = torch.empty((m, 8), dtype=torch.float32, device="cuda")
+        rnd[:, 0:3] = torch.rand((m, 3), generator=g, device="cuda") * N
+        rnd[:, 3] = 0.0
+        rnd[:, 4:7] = torch.randn((m, 3), generator=g, device="cuda")
+        rnd[:, 7] = 3.0e38
+        dev_rays.append(torch.cat([cam_rays, rnd]))
+        total += n
+        offsets.append(total)
+    rays = torch.cat(dev_rays).contiguous().view(torch.uint8).view(-1)
+    del dev_rays
+    torch.cuda.synchronize()
+    log(f"[bench] batched set: 64 scenes, {total} rays, built in {time.perf_counter() - t0:.1f}s")
+    offs = np.array(offsets, dtype=np.uint64)
+    hits_plain = torch.empty_like(rays)
+    hits_batch = torch.empty_like(rays)
+
+    # per-scene launch arguments prepared up front so the loop is just 64 C calls
+    rp, hp, sp = rays.data_ptr(), hits_plain.data_ptr(), stream.cuda_stream
+    plain_args = [(sc._h, rp + int(offs[s]) * 32, int(offs[s + 1] - offs[s]), hp + int(offs[s]) * 32)
+                  for s, sc in enumerate(scenes)]
+    trace = vx.LIB.vx_trace
+
+    def plain():
+        for h_, r_, n_, o_ in plain_args:
+            if trace(h_, r_, n_, o_, sp) != 0:
+                vx._check(1, "vx_trace")
+
+    batch = vx.Batch(ctx, scenes)
+    tuning, tune_ms = batch.autotune(rays, offs, hits_batch)
+    log(f"[bench] autotuned {tuning.to_dict()} ({tune_ms:.3f} ms)")
+    ms_plain = timed(torch, stream, plain, steps, 2)
+    ms_batch = timed(torch, stream, lambda: batch.trace(rays, offs, hits_batch), steps, 2)
+    graph = batch.graph(rays, offs, hits_batch)
+    ms_graph = timed(torch, stream, lambda: graph.launch(), steps, 2)
+    identical = bool(torch.equal(hits_plain, hits_batch))
+    graph.close()
+    batch.close()
+    for sc in scenes:
+        sc.close()
+    return {"workload": "64 scenes cave/urban, edges {64..512}, occupancy 10-90%, BC1, 1e4-2e6 rays each "
+                        "(half pinhole, half random-direction)",
+            "rays": total, "scene_edges": {str(e): edges.count(e) for e in (64, 128, 256, 512)},
+            "plain_ms": round(ms_plain, 4), "batched_ms": round(ms_batch, 4), "graph_ms": round(ms_graph, 4),
+            "mrays_plain": round(total / ms_plain / 1e3, 1), "mrays_batched": round(total / ms_batch / 1e3, 1),
+            "speedup": round(ms_plain / ms_batch, 3), "tuning": tuning.to_dict(), "bit_identical": identical}
+
+
+# ---------------------------------------------------------------------------
+# fwd+bwd leg
+# ---------------------------------------------------------------------------
+def fwd_bwd_leg(vx, torch, ctx, stream, grid, built, steps: int):
+    scene = vx.Scene(ctx, built)
+    V = scene.voxel_count()
+    rng = np.random.default_rng(55)
+    light = rng.normal(size=3)
+    light = (light / np.linalg.norm(light)).astype(np.float32)
+    albedo = torch.from_numpy(rng.random((V, 3), dtype=np.float32)).cuda()
+    target_albedo = torch.from_numpy(np.random.default_rng(56).random((V, 3), dtype=np.float32)).cuda()
+    views = vx.seeded_views(grid, 57, 4)
+    W, H = 1280, 720
+    targets = [scene.albedo_forward(c, W, H, light, target_albedo, store_hits=False)[0] for c in views]
+    grad = torch.zeros((V, 3), dtype=torch.float32, device="cuda")
+    imgs = [torch.empty((H, W, 3), dtype=torch.float32, device="cuda") for _ in views]
+    rows = [torch.empty(H * W, dtype=torch.int32, device="cuda") for _ in views]
+    shades = [torch.empty(H * W, dtype=torch.float32, device="cuda") for _ in views]
+    gimg = [torch.empty((H, W, 3), dtype=torch.float32, device="cuda") for _ in views]
+    lt = light.ctypes.data
+    cams = [vx.C.byref(c) for c in views]
+    hp = [torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")]
+
+    def plain_fwd():
+        for c in views:
+            vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(c), W, H, 3.0e38, hp[0].data_ptr(), None,
+                                       stream.cuda_stream), "vx_render")
+
+    def fwd():
+        for i, c in enumerate(cams):
+            vx._check(vx.LIB.vx_albedo_forward(scene._h, c, W, H, lt, albedo.data_ptr(), imgs[i].data_ptr(),
+                                               rows[i].data_ptr(), shades[i].data_ptr(), stream.cuda_stream),
+                      "vx_albedo_forward")
+
+    def loss_grad():
+        for i in range(len(views)):
+            torch.sub(imgs[i], targets[i], out=gimg[i])
+            gimg[i].mul_(2.0)
+
+    def bwd(mode, stored=True):
+        def run():
+            for i, c in enumerate(cams):
+                vx._check(vx.LIB.vx_albedo_backward(scene._h, c, W, H, lt, gimg[i].data_ptr(),
+                                                    rows[i].data_ptr() if stored else None,
+                                                    shades[i].data_ptr() if stored else None, grad.data_ptr(), mode,
+                                                    stream.cuda_stream), "vx_albedo_backward")
+        return run
+
+    fwd()
+    loss_grad()
+    torch.cuda.synchronize()
+    n = len(views)
+    ms_plain = timed(torch, stream, plain_fwd, steps, 2) / n
+    ms_fwd = timed(torch, stream, fwd, steps, 2) / n
+    ms_bwd_det = timed(torch, stream, bwd(1), steps, 2) / n
+    ms_bwd_atomic = timed(torch, stream, bwd(0), steps, 2) / n
+    ms_bwd_recompute = timed(torch, stream, bwd(1, stored=False), steps, 2) / n
+    scene.close()
+    return {"workload": "cave256 albedo fit, 4 seeded 1280x720 views, L2 loss, one directional light",
+            "voxels": V, "plain_forward_ms_per_view": round(ms_plain, 4), "forward_ms_per_view": round(ms_fwd, 4),
+            "backward_ms_per_view": round(ms_bwd_det, 4), "backward_mode": "deterministic, stored hits",
+            "backward_atomic_ms_per_view": round(ms_bwd_atomic, 4),
+            "backward_recompute_ms_per_view": round(ms_bwd_recompute, 4),
+            "bwd_over_fwd": round(ms_bwd_det / ms_fwd, 3),
+            "fwd_bwd_over_plain_forward": round((ms_fwd + ms_bwd_det) / ms_plain, 3)}
 
 
 def main() -> int:
@@ -71,6 +235,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched leg")
+    ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -101,20 +267,9 @@ def main() -> int:
         vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(cam), W, H, 1e30, hits.data_ptr(), None,
                                    stream.cuda_stream), "vx_render")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     wall0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    ms_step = timed(torch, stream, step, args.steps, args.warmup)
     wall = time.perf_counter() - wall0
-    ms_total = ev0.elapsed_time(ev1)
-    ms_step = ms_total / args.steps
     rays = W * H
     mrays = rays / (ms_step * 1e-3) / 1e6
     log(f"[bench] {args.steps} launches: {ms_step:.4f} ms/launch, {mrays:.1f} Mrays/s (wall {wall:.3f}s)")
@@ -148,12 +303,21 @@ def main() -> int:
                    "bricks": built.brick_count, "descriptors": built.node_count, "colour": "bc1"},
         "roofline": roofline,
     }
+    got = vx.hits_to_numpy(hits)
+    if not args.no_fwdbwd:
+        log("[bench] fwd+bwd leg...")
+        result["fwd_bwd"] = fwd_bwd_leg(vx, torch, ctx, stream, grid, built, max(3, args.steps // 5))
+        log(f"[bench] fwd_bwd: {result['fwd_bwd']}")
+    scene.close()
+    if not args.no_batched:
+        log("[bench] batched leg...")
+        result["batched"] = batched_leg(vx, torch, ctx, stream, max(3, args.steps // 10))
+        log(f"[bench] batched: {result['batched']}")
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle on host cores)...")
         oray = orc.camera_rays(cam.as_floats(), W, H, 1e30)
         base, want = cpu_baseline(orc, built, oray)
         result["cpu_baseline"] = base
-        got = vx.hits_to_numpy(hits)
         diff = int((got.view(np.uint32).reshape(-1, 8) != want.view(np.uint32).reshape(-1, 8)).any(1).sum())
         result["parity"] = "bit-exact vs oracle" if diff == 0 else f"{diff} records differ from the oracle"
     else:

@@ -51,7 +51,8 @@ def c5(vx, orc, cave, gpu_ctx):
     for cam in views:
         target, _, _ = scene.albedo_forward(cam, W, H, light, target_albedo, store_hits=False)
         hits, _ = orc.trace(oscene, orc.camera_rays(cam.as_floats(), W, H, 3.0e38), nthreads=8)
-        data.append({"cam": cam, "target": target, "rows": voxel_rows(built, hits), "shade": shades(hits, light)})
+        data.append({"cam": cam, "target": target, "target_np": target.cpu().numpy().reshape(-1, 3),
+                     "rows": voxel_rows(built, hits), "shade": shades(hits, light)})
     torch.cuda.synchronize()
     return {"scene": scene, "built": built, "V": V, "light": light, "albedo": albedo, "views": data}
 
@@ -63,8 +64,7 @@ def test_forward_matches_oracle_hits(vx, c5):
         img, rows, shade = c5["scene"].albedo_forward(v["cam"], W, H, c5["light"], c5["albedo"])
         torch.cuda.synchronize()
         r = v["rows"]
-        assert np.array_equal(rows.cpu().numpy().astype(np.int64), np.where(r < 0, 0xFFFFFFFF, r).astype(np.int64)
-                              .astype(np.uint32).astype(np.int32).astype(np.int64))
+        assert np.array_equal(rows.cpu().numpy(), r.astype(np.int32))  # a miss is row ~0u, i.e. -1 as int32
         assert shade.cpu().numpy().tobytes() == v["shade"].tobytes()
         want = np.where((r >= 0)[:, None], a[np.maximum(r, 0)] * v["shade"][:, None], np.float32(0)).astype(np.float32)
         assert img.cpu().numpy().reshape(-1, 3).tobytes() == want.tobytes()
@@ -83,18 +83,30 @@ def _gpu_grad(vx, c5, deterministic=True, stored=True):
     return grad.cpu().numpy()
 
 
+def _cpu_reference(c5):
+    """CPU closed form from the oracle's hits: dL/da[v] = sum over pixels of 2 (pred - target) shade,
+    with pred the FP32 forward value (albedo * shade rounded to FP32, as the forward test pins it)
+    and the sum taken in FP64. Also returns each voxel's sum of |contribution|, the scale that
+    FP32 accumulation error is relative to."""
+    a32 = c5["albedo"].cpu().numpy()
+    V = c5["V"]
+    cpu = np.zeros((V, 3))
+    mag = np.zeros((V, 3))
+    for v in c5["views"]:
+        r, s = v["rows"], v["shade"].astype(np.float64)
+        t = v["target_np"].astype(np.float64)
+        hit = r >= 0
+        pred = (a32[r[hit]] * v["shade"][hit, None]).astype(np.float64)
+        term = 2.0 * (pred - t[hit]) * s[hit, None]
+        np.add.at(cpu, r[hit], term)
+        np.add.at(mag, r[hit], np.abs(term))
+    return cpu, mag
+
+
 def test_gradient_vs_fd_and_cpu_reference(vx, c5):
     a = c5["albedo"].cpu().numpy().astype(np.float64)
     grad = _gpu_grad(vx, c5)
-    V = c5["V"]
-    # CPU closed form from the oracle's hits: dL/da[v] = sum over pixels of 2 (pred - target) shade
-    cpu = np.zeros((V, 3))
-    for v in c5["views"]:
-        r, s = v["rows"], v["shade"].astype(np.float64)
-        t = v["target"].cpu().numpy().reshape(-1, 3).astype(np.float64)
-        hit = r >= 0
-        pred = a[r[hit]] * s[hit, None]
-        np.add.at(cpu, r[hit], 2.0 * (pred - t[hit]) * s[hit, None])
+    cpu, mag = _cpu_reference(c5)
     # FP64 central differences on 256 sampled visible voxels, eps = 1e-3
     rng = np.random.default_rng(3)
     visible = np.unique(np.concatenate([v["rows"][v["rows"] >= 0] for v in c5["views"]]))
@@ -107,17 +119,20 @@ def test_gradient_vs_fd_and_cpu_reference(vx, c5):
                 for v in c5["views"]:
                     m = v["rows"] == vox
                     s = v["shade"][m].astype(np.float64)
-                    t = v["target"].cpu().numpy().reshape(-1, 3)[m, ch].astype(np.float64)
+                    t = v["target_np"][m, ch].astype(np.float64)
                     total += np.sum(((a[vox, ch] + delta) * s - t) ** 2)
                 return total
             fd = (loss(eps) - loss(-eps)) / (2 * eps)
             scale = max(abs(fd), 1e-3)
             assert abs(grad[vox, ch] - fd) <= 1e-4 * scale, (vox, ch, grad[vox, ch], fd)
-            assert abs(cpu[vox, ch] - fd) <= 1e-9 * max(abs(fd), 1.0)
-    nz = np.abs(cpu) > 1e-3
-    rel = np.abs(grad[nz] - cpu[nz]) / np.abs(cpu[nz])
-    assert rel.max() <= 1e-4
-    assert np.abs(grad[~nz] - cpu[~nz]).max() <= 1e-6
+            assert abs(cpu[vox, ch] - fd) <= 1e-4 * scale
+    # every voxel: error within 1e-5 of its summed contribution magnitude; voxels whose
+    # contributions do not cancel (|sum| >= 0.1 * magnitude): within 1e-4 relative
+    err = np.abs(grad - cpu)
+    assert (err <= 1e-5 * mag + 1e-7).all(), float((err / (mag + 1e-30)).max())
+    ok = (mag > 0) & (np.abs(cpu) >= 0.1 * mag)
+    assert ok.sum() > 0.8 * (mag > 0).sum()  # most visible voxel channels do not cancel
+    assert (err[ok] / np.abs(cpu[ok])).max() <= 1e-4
 
 
 def test_deterministic_is_reproducible_and_matches_atomic(vx, c5):
@@ -125,7 +140,17 @@ def test_deterministic_is_reproducible_and_matches_atomic(vx, c5):
     g2 = _gpu_grad(vx, c5, deterministic=True)
     assert g1.tobytes() == g2.tobytes()
     ga = _gpu_grad(vx, c5, deterministic=False)
-    assert np.allclose(ga, g1, rtol=1e-5, atol=1e-6)
+    cpu, mag = _cpu_reference(c5)
+    # the two reductions differ only in summation order
+    err = np.abs(ga - g1)
+    bad = err > 1e-5 * mag + 1e-7
+    if bad.any():
+        i = np.unravel_index(np.argmax(err / (mag + 1e-30)), err.shape)
+        low = int((np.abs(ga[bad]) < np.abs(g1[bad])).sum())
+        raise AssertionError(f"{int(bad.sum())} entries ({low} smaller in magnitude); worst {i}: atomic {ga[i]!r} "
+                             f"deterministic {g1[i]!r} "
+                             f"cpu {cpu[i]!r} magnitude {mag[i]!r}; atomic vs cpu max "
+                             f"{float((np.abs(ga - cpu) / (mag + 1e-30)).max())}")
 
 
 def test_recomputed_hits_equal_stored(vx, c5):

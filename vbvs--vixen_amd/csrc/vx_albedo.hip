@@ -141,8 +141,23 @@ __global__ void __launch_bounds__(kAlbBlock)
     }
 }
 
-// Per-pixel contribution dL/dpixel * shade. ATOMIC adds it straight into grad;
-// otherwise it emits a (row, contribution) pair for the sorted reduction.
+// Sum of v over the lanes in `group` (a subset of the active lanes), broadcast to all lanes.
+__device__ __forceinline__ float group_sum(float v, uint64_t group, uint32_t lane)
+{
+    float x = ((group >> lane) & 1ull) ? v : 0.0f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Per-pixel contribution dL/dpixel * shade. ATOMIC adds it into grad with wave
+// aggregation: lanes that hit the same row are summed across the wave first, so
+// a voxel near the camera covering thousands of pixels receives one hardware
+// FP32 atomic per wave instead of one per pixel (fewer contended atomics and
+// less rounding in the atomic unit); once the next group is a single lane the
+// remaining lanes add directly. Otherwise it emits (row, contribution) pairs
+// for the sorted reduction. Every lane runs to the end (no early exit), so the
+// cross-lane operations see the whole wave.
 template <bool RECOMPUTE, bool ATOMIC>
 __global__ void __launch_bounds__(kAlbBlock)
     k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
@@ -152,25 +167,47 @@ __global__ void __launch_bounds__(kAlbBlock)
 {
     if (RECOMPUTE) stage_alb(s);
     uint32_t px, py;
-    if (!tile_pixel(w, h, px, py)) return;
+    const bool inside = tile_pixel(w, h, px, py);
     const uint64_t p = (uint64_t)py * w + px;
-    uint32_t row;
-    float shade;
-    if (RECOMPUTE) {
-        row = trace_pixel(s, c, base, px, py, w, h, shade);
-    } else {
-        row = hit_row[p];
-        shade = hit_shade[p];
+    uint32_t row = kNoVoxel;
+    float shade = 0.0f;
+    if (inside) {
+        if (RECOMPUTE) {
+            row = trace_pixel(s, c, base, px, py, w, h, shade);
+        } else {
+            row = hit_row[p];
+            shade = hit_shade[p];
+        }
     }
-    const float gr = grad_image[p * 3 + 0] * shade, gg = grad_image[p * 3 + 1] * shade,
-                gb = grad_image[p * 3 + 2] * shade;
+    float gr = 0.0f, gg = 0.0f, gb = 0.0f;
+    if (inside) {
+        gr = grad_image[p * 3 + 0] * shade;
+        gg = grad_image[p * 3 + 1] * shade;
+        gb = grad_image[p * 3 + 2] * shade;
+    }
     if (ATOMIC) {
-        if (row != kNoVoxel) {
+        const uint32_t lane = threadIdx.x & 63u;
+        uint64_t pending = __ballot(row != kNoVoxel);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const uint32_t lrow = __shfl(row, leader, 64);
+            const uint64_t group = __ballot(row == lrow) & pending;
+            if ((group & (group - 1)) == 0ull) break; // a single lane: stop aggregating
+            const float sr = group_sum(gr, group, lane), sg = group_sum(gg, group, lane),
+                        sb = group_sum(gb, group, lane);
+            if ((int)lane == leader) {
+                atomicAdd(&grad[(size_t)lrow * 3 + 0], sr);
+                atomicAdd(&grad[(size_t)lrow * 3 + 1], sg);
+                atomicAdd(&grad[(size_t)lrow * 3 + 2], sb);
+            }
+            pending &= ~group;
+        }
+        if ((pending >> lane) & 1ull) {
             atomicAdd(&grad[(size_t)row * 3 + 0], gr);
             atomicAdd(&grad[(size_t)row * 3 + 1], gg);
             atomicAdd(&grad[(size_t)row * 3 + 2], gb);
         }
-    } else {
+    } else if (inside) {
         keys[p] = row;
         vals[p] = Rgb{gr, gg, gb};
     }
