@@ -188,6 +188,14 @@ vx_status vx_render(vx_scene* scene, const vx_camera* camera, uint32_t width, ui
 vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t height, float tmax,
                          vx_ray* rays, vx_stream stream);
 
+/* One seeded cosine-weighted diffuse bounce per primary hit (the urban
+ * workload's secondary rays; no reference counterpart): origin on the hit
+ * voxel's entry face, 1e-3 voxel off the surface along its normal, direction
+ * cosine-distributed about the normal from a hash of (seed, ray index). A
+ * missed primary gets a zero direction (a miss when traced). n device rays. */
+vx_status vx_bounce_rays(const vx_scene* scene, const vx_ray* primary, const vx_hit* hits, uint64_t n,
+                         uint32_t seed, vx_ray* out, vx_stream stream);
+
 /* Per-launch traversal counters for the algorithmic-bytes model:
  * counters[0] rays, [1] ESVO iterations (8-byte descriptor fetch each),
  * [2] brick visits (8-byte leaf descriptor fetch each), [3] DDA voxels examined

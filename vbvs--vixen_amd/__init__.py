@@ -127,6 +127,7 @@ def _load() -> C.CDLL:
         "vx_trace_counters": ([vp, vp, u64, vp, vp, vp], st),
         "vx_render": ([vp, C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp, vp], st),
         "vx_camera_rays": ([C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp], st),
+        "vx_bounce_rays": ([vp, vp, vp, u64, u32, vp, vp], st),
         "vx_build_from_dense": ([vp, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
@@ -160,7 +161,7 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
-            "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward")
+            "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays")
 
 
 def _check(status: int, where: str) -> None:
@@ -379,6 +380,15 @@ class Scene:
                              ib.data_ptr() if ib is not None else None, _stream_ptr(stream)), "vx_render")
         return hb, ib
 
+
+    def bounce_rays(self, primary, hits, seed: int, stream=None):
+        """Seeded cosine-weighted diffuse bounce rays for the primary hits (device buffers of n x 32 bytes)."""
+        torch = _torch()
+        n = primary.numel() * primary.element_size() // 32
+        out = torch.empty(n * 32, dtype=torch.uint8, device=primary.device)
+        _check(LIB.vx_bounce_rays(self._h, primary.data_ptr(), hits.data_ptr(), n, seed & 0xFFFFFFFF, out.data_ptr(),
+                                  _stream_ptr(stream)), "vx_bounce_rays")
+        return out
 
     def set_lds_depth(self, depth: int) -> None:
         _check(LIB.vx_scene_set_lds_depth(self._h, depth), "vx_scene_set_lds_depth")

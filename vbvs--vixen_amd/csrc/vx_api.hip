@@ -181,6 +181,59 @@ __global__ void __launch_bounds__(kBlock) k_camera_rays(CamParams c, uint32_t w,
     dst[1] = make_float4(d[0], d[1], d[2], tmax);
 }
 
+// One cosine-weighted diffuse bounce per primary hit: origin on the entry face
+// of the hit voxel, pushed 1e-3 voxel off the surface along the face normal,
+// direction sqrt(u1)-distributed about the normal (Malley's method) from a
+// counter-based hash of (seed, ray index). Misses get a zero direction, which
+// vx_trace reports as a miss.
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bounce_rays(const vx_ray* __restrict__ prim, const vx_hit* __restrict__ hits,
+                                                        uint64_t n, uint32_t seed, float k,
+                                                        vx_ray* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float o[3], d[3], tmin, tmax;
+    load_ray(prim, i, o, d, tmin, tmax);
+    const vx_hit h = hits[i];
+    float4* dst = reinterpret_cast<float4*>(out + i);
+    if (!(h.flags & 1u)) {
+        dst[0] = make_float4(o[0], o[1], o[2], 0.0f);
+        dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float inv = 1.0f / __builtin_sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const uint32_t axis = (h.flags >> 1) & 3u;
+    const float sgn = (h.flags & 8u) ? 1.0f : -1.0f;
+    float nrm[3] = {0.0f, 0.0f, 0.0f};
+    nrm[axis] = sgn;
+    const float eps = 1e-3f / k; // 1e-3 voxel in world units
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = o[a] + d[a] * inv * h.t + nrm[a] * eps;
+    const uint32_t r1 = mix32((uint32_t)i * 2u + 1u + mix32(seed ^ (uint32_t)(i >> 32)));
+    const uint32_t r2 = mix32(r1 ^ 0x9e3779b9u);
+    const float u1 = (float)(r1 >> 8) * 0x1p-24f, u2 = (float)(r2 >> 8) * 0x1p-24f;
+    const float r = __builtin_sqrtf(u1), phi = 6.28318530717958648f * u2;
+    const float a1 = r * __cosf(phi), a2 = r * __sinf(phi), up = __builtin_sqrtf(1.0f - u1);
+    const uint32_t t1 = (axis + 1u) % 3u, t2 = (axis + 2u) % 3u;
+    float dir[3];
+    dir[axis] = sgn * up;
+    dir[t1] = a1;
+    dir[t2] = a2;
+    dst[0] = make_float4(p[0], p[1], p[2], 0.0f);
+    dst[1] = make_float4(dir[0], dir[1], dir[2], 3.0e38f);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -491,6 +544,17 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     vx::k_render<<<dim3(per * vx::kXcds), dim3(vx::kBlock), vx::launch_lds(v), st>>>(
         v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
+    return VX_OK;
+}
+
+vx_status vx_bounce_rays(const vx_scene* s, const vx_ray* primary, const vx_hit* hits, uint64_t n, uint32_t seed,
+                         vx_ray* out, vx_stream stream)
+{
+    if (!s || (n && (!primary || !hits || !out))) return vx::fail(VX_ERR_INVALID_ARG, "vx_bounce_rays: null argument");
+    if (n == 0) return VX_OK;
+    vx::k_bounce_rays<<<dim3((unsigned)((n + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0,
+                        static_cast<hipStream_t>(stream)>>>(primary, hits, n, seed, s->view.k, out);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_bounce_rays: launch failed");
     return VX_OK;
 }
 
