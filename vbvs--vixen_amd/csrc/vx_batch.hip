@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kMaxGroup)
                     } else {
                         unsigned int v = 0u;
                         if (lane == 0u) v = atomicAdd(queue, 1u);
-                        c = (uint64_t)__shfl(v, 0, 64);
+                        c = (uint64_t)__builtin_amdgcn_readfirstlane(v); // lane 0: the whole wave is here
                         if (c >= n_chunks) {
                             drained = true;
                             break;

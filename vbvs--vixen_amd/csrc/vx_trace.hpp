@@ -22,7 +22,11 @@
 //   - brick occupancy is read from a 64-byte-per-brick bit mask derived from
 //     brickData at upload (bit lin of the brick's 16 words), so a brick's DDA
 //     touches one cache line instead of 2 KiB; the hit voxel's brickData word
-//     is read only when the scene has no BC1 colours.
+//     is read only when the scene has no BC1 colours;
+//   - the brick DDA is software-pipelined: the voxel sequence does not depend
+//     on occupancy, so each step computes the next voxel and issues its mask
+//     load before testing the current voxel (one load in flight per lane);
+//     the sequence of voxels tested, and so the result, is unchanged.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -224,12 +228,33 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
             tNext[i] = btmin + (entry[i] - nb) / __builtin_fabsf(dl[i]);
         }
     }
+    // cur starts inside the brick (clamped above). Each pass steps the DDA first
+    // (the reference steps after the test: the same voxel order, the same exits at
+    // t > btmax, outside the brick or after 24 voxels) and loads the next voxel's
+    // mask word, always from inside this brick, before it tests the current one.
     const uint32_t* occ = s.occ + (size_t)brick * 16u;
-    for (int it = 0; it < 24; ++it) {
-        if ((uint32_t)cur[0] > 7u || (uint32_t)cur[1] > 7u || (uint32_t)cur[2] > 7u) return false;
-        const uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
+    uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
+    uint32_t word = occ[lin >> 5];
+    for (int it = 0;; ++it) {
         if (COUNT) cnt.voxels += 1;
-        const uint32_t word = occ[lin >> 5];
+        int n0 = cur[0], n1 = cur[1], n2 = cur[2];
+        bool stop;
+        if (tNext[0] < tNext[1] && tNext[0] < tNext[2]) {
+            stop = tNext[0] > btmax;
+            n0 += step[0];
+            tNext[0] += tDelta[0];
+        } else if (tNext[1] < tNext[2]) {
+            stop = tNext[1] > btmax;
+            n1 += step[1];
+            tNext[1] += tDelta[1];
+        } else {
+            stop = tNext[2] > btmax;
+            n2 += step[2];
+            tNext[2] += tDelta[2];
+        }
+        const bool more = !stop && it + 1 < 24 && (uint32_t)n0 <= 7u && (uint32_t)n1 <= 7u && (uint32_t)n2 <= 7u;
+        const uint32_t nlin = (uint32_t)((n0 & 7) + 8 * (n1 & 7) + 64 * (n2 & 7));
+        const uint32_t nword = occ[nlin >> 5];
         if ((word >> (lin & 31u)) & 1u) {
             float tnh[3];
 #pragma unroll
@@ -268,21 +293,13 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
             else rgba8_decode(s.brick_data[(size_t)brick * 512u + lin], h.rgb);
             return true;
         }
-        if (tNext[0] < tNext[1] && tNext[0] < tNext[2]) {
-            if (tNext[0] > btmax) return false;
-            cur[0] += step[0];
-            tNext[0] += tDelta[0];
-        } else if (tNext[1] < tNext[2]) {
-            if (tNext[1] > btmax) return false;
-            cur[1] += step[1];
-            tNext[1] += tDelta[1];
-        } else {
-            if (tNext[2] > btmax) return false;
-            cur[2] += step[2];
-            tNext[2] += tDelta[2];
-        }
+        if (!more) return false;
+        cur[0] = n0;
+        cur[1] = n1;
+        cur[2] = n2;
+        lin = nlin;
+        word = nword;
     }
-    return false;
 }
 
 // Descriptor fetch: a prefix of the descriptor array (the top levels of a
