@@ -274,7 +274,8 @@ vx_status vx_albedo_forward(vx_scene* scene, const vx_camera* camera, uint32_t w
 /* Adjoint: grad_albedo[row] += grad_image[p] * shade[p] over the pixels p that
  * hit the row. hit_row/hit_shade from vx_albedo_forward, or NULL to re-trace the
  * hits. mode VX_GRAD_ATOMIC (FP32 atomics) or VX_GRAD_DETERMINISTIC (stable sort
- * by row + reduce by key: identical bits on every run). grad_albedo accumulates. */
+ * by row, then each row's run summed in a fixed order: identical bits on every
+ * run; needs width*height < 2^31). grad_albedo accumulates. */
 vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
                              const float light[3], const float* grad_image, const uint32_t* hit_row,
                              const float* hit_shade, float* grad_albedo, uint32_t mode, vx_stream stream);

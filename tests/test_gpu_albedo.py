@@ -137,8 +137,9 @@ def test_gradient_vs_fd_and_cpu_reference(vx, c5):
 
 def test_deterministic_is_reproducible_and_matches_atomic(vx, c5):
     g1 = _gpu_grad(vx, c5, deterministic=True)
-    g2 = _gpu_grad(vx, c5, deterministic=True)
-    assert g1.tobytes() == g2.tobytes()
+    for _ in range(3):  # runs that span many pixels (voxels near the camera) must not depend on timing
+        g2 = _gpu_grad(vx, c5, deterministic=True)
+        assert g1.tobytes() == g2.tobytes()
     ga = _gpu_grad(vx, c5, deterministic=False)
     cpu, mag = _cpu_reference(c5)
     # the two reductions differ only in summation order

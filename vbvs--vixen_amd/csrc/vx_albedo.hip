@@ -15,9 +15,14 @@
 // recomputed by re-tracing (no per-pixel storage). Two reductions:
 //   atomic         one FP32 atomic add per hit pixel and channel;
 //   deterministic  (row, contribution) pairs are radix-sorted by row (stable,
-//                  so equal rows keep pixel order), reduced by key, and the
-//                  per-row sums added without conflicts: identical bits on
-//                  every run.
+//                  so equal rows keep pixel order); the run heads are selected
+//                  (integer work) and one wave sums each run in a fixed order
+//                  (lane l takes elements l, l+64, ..., then a fixed butterfly),
+//                  so the per-row sums, added without conflicts, depend only on
+//                  the input: identical bits on every run. (A decoupled
+//                  look-back reduce-by-key is not used: the order in which it
+//                  combines the partial sums of a run spanning several tiles
+//                  depends on timing.)
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -34,9 +39,6 @@ constexpr uint32_t kNoVoxel = 0xFFFFFFFFu;
 
 struct Rgb {
     float r, g, b;
-};
-struct RgbSum {
-    __host__ __device__ Rgb operator()(const Rgb& a, const Rgb& b) const { return Rgb{a.r + b.r, a.g + b.g, a.b + b.b}; }
 };
 
 struct AlbCam {
@@ -213,18 +215,49 @@ __global__ void __launch_bounds__(kAlbBlock)
     }
 }
 
-__global__ void __launch_bounds__(kAlbBlock) k_scatter_runs(const uint32_t* __restrict__ rows,
-                                                           const Rgb* __restrict__ sums,
-                                                           const uint32_t* __restrict__ n_runs,
-                                                           float* __restrict__ grad)
+// Run heads of the sorted keys: index i starts a run when its key differs from
+// the previous one.
+struct RunHead {
+    const uint32_t* keys;
+    __host__ __device__ bool operator()(uint32_t i) const { return i == 0u || keys[i] != keys[i - 1u]; }
+};
+
+// One wave per run (grid-stride over the runs): lane l adds elements l, l+64,
+// ... of the run in order, a fixed butterfly combines the lanes and lane 0 adds
+// the sum into the run's row (rows are unique across runs: no conflicts). The
+// last run of misses (key ~0u) is skipped.
+__global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __restrict__ keys_sorted,
+                                                          const Rgb* __restrict__ vals_sorted,
+                                                          const uint32_t* __restrict__ heads,
+                                                          const uint32_t* __restrict__ n_runs, uint32_t n_items,
+                                                          float* __restrict__ grad)
 {
-    const uint32_t i = blockIdx.x * kAlbBlock + threadIdx.x;
-    if (i >= *n_runs) return;
-    const uint32_t row = rows[i];
-    if (row == kNoVoxel) return;
-    grad[(size_t)row * 3 + 0] += sums[i].r;
-    grad[(size_t)row * 3 + 1] += sums[i].g;
-    grad[(size_t)row * 3 + 2] += sums[i].b;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = (gridDim.x * kAlbBlock) >> 6;
+    const uint32_t n = *n_runs;
+    for (uint32_t r = (blockIdx.x * kAlbBlock + threadIdx.x) >> 6; r < n; r += waves) {
+        const uint32_t b = heads[r], e = r + 1u < n ? heads[r + 1u] : n_items;
+        const uint32_t row = keys_sorted[b];
+        if (row == kNoVoxel) continue; // the same r on every lane: uniform
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        for (uint32_t i = b + lane; i < e; i += 64u) {
+            const Rgb v = vals_sorted[i];
+            sr += v.r;
+            sg += v.g;
+            sb += v.b;
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            sr += __shfl_xor(sr, m, 64);
+            sg += __shfl_xor(sg, m, 64);
+            sb += __shfl_xor(sb, m, 64);
+        }
+        if (lane == 0u) {
+            grad[(size_t)row * 3 + 0] += sr;
+            grad[(size_t)row * 3 + 1] += sg;
+            grad[(size_t)row * 3 + 2] += sb;
+        }
+    }
 }
 
 } // namespace vx
@@ -235,8 +268,8 @@ struct vx_albedo_state {
     uint64_t voxels = 0;
     // sorted-reduction workspace, grown on demand
     size_t cap_px = 0, cap_tmp = 0;
-    uint32_t *keys = nullptr, *keys_sorted = nullptr, *rows = nullptr, *n_runs = nullptr;
-    vx::Rgb *vals = nullptr, *vals_sorted = nullptr, *sums = nullptr;
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *heads = nullptr, *n_runs = nullptr;
+    vx::Rgb *vals = nullptr, *vals_sorted = nullptr;
     void* tmp = nullptr;
 };
 
@@ -311,15 +344,14 @@ static SceneView alb_view(const vx_scene* s, uint32_t& lds)
 static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
 {
     if (px > a->cap_px) {
-        for (void* p : {(void*)a->keys, (void*)a->keys_sorted, (void*)a->rows, (void*)a->n_runs, (void*)a->vals,
-                        (void*)a->vals_sorted, (void*)a->sums})
+        for (void* p : {(void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs, (void*)a->vals,
+                        (void*)a->vals_sorted})
             (void)hipFree(p);
         a->cap_px = 0;
         if (hipMalloc(&a->keys, px * 4) != hipSuccess || hipMalloc(&a->keys_sorted, px * 4) != hipSuccess ||
-            hipMalloc(&a->rows, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 4) != hipSuccess ||
+            hipMalloc(&a->heads, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 4) != hipSuccess ||
             hipMalloc(&a->vals, px * sizeof(Rgb)) != hipSuccess ||
-            hipMalloc(&a->vals_sorted, px * sizeof(Rgb)) != hipSuccess ||
-            hipMalloc(&a->sums, px * sizeof(Rgb)) != hipSuccess)
+            hipMalloc(&a->vals_sorted, px * sizeof(Rgb)) != hipSuccess)
             return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
         a->cap_px = px;
     }
@@ -337,8 +369,8 @@ static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
 void vx_albedo_state_free(vx_albedo_state* a)
 {
     if (!a) return;
-    for (void* p : {(void*)a->base, (void*)a->keys, (void*)a->keys_sorted, (void*)a->rows, (void*)a->n_runs,
-                    (void*)a->vals, (void*)a->vals_sorted, (void*)a->sums, a->tmp})
+    for (void* p : {(void*)a->base, (void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs,
+                    (void*)a->vals, (void*)a->vals_sorted, a->tmp})
         (void)hipFree(p);
     delete a;
 }
@@ -405,13 +437,15 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
         if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
         return VX_OK;
     }
-    // deterministic: sort (row, contribution) by row, reduce by key, scatter the run sums
+    // deterministic: sort (row, contribution) by row, select the run heads, sum each run in a fixed order
+    if (px > 0x7FFFFFFFu) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: deterministic mode needs w*h < 2^31");
     const int end_bit = 32; // misses carry the all-ones key and sort last
     size_t sort_bytes = 0, red_bytes = 0;
+    const hipcub::CountingInputIterator<uint32_t> index(0u);
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (vx::Rgb*)nullptr, (vx::Rgb*)nullptr, (int)px, 0, end_bit, st) != hipSuccess ||
-        hipcub::DeviceReduce::ReduceByKey(nullptr, red_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (vx::Rgb*)nullptr,
-                                          (vx::Rgb*)nullptr, (uint32_t*)nullptr, vx::RgbSum(), (int)px, st) != hipSuccess)
+        hipcub::DeviceSelect::If(nullptr, red_bytes, index, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px,
+                                 vx::RunHead{nullptr}, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: workspace query failed");
     if (vx_status g = vx::grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
     if (recompute)
@@ -426,12 +460,15 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
                                            end_bit, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: sort failed");
     tb = a->cap_tmp;
-    if (hipcub::DeviceReduce::ReduceByKey(a->tmp, tb, a->keys_sorted, a->rows, a->vals_sorted, a->sums, a->n_runs,
-                                          vx::RgbSum(), (int)px, st) != hipSuccess)
-        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: reduce failed");
-    vx::k_scatter_runs<<<dim3((unsigned)((px + vx::kAlbBlock - 1) / vx::kAlbBlock)), block, 0, st>>>(
-        a->rows, a->sums, a->n_runs, grad_albedo);
-    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: scatter failed");
+    if (hipcub::DeviceSelect::If(a->tmp, tb, index, a->heads, a->n_runs, (int)px, vx::RunHead{a->keys_sorted}, st) !=
+        hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run selection failed");
+    // a few resident groups per CU walk all runs (their count stays on the device)
+    const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->ctx->cu_count * 8u,
+                                                                               (px + 3) / 4));
+    vx::k_reduce_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals_sorted, a->heads, a->n_runs,
+                                                      (uint32_t)px, grad_albedo);
+    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run reduction failed");
     return VX_OK;
 }
 

@@ -232,74 +232,78 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     // (the reference steps after the test: the same voxel order, the same exits at
     // t > btmax, outside the brick or after 24 voxels) and loads the next voxel's
     // mask word, always from inside this brick, before it tests the current one.
+    // The step is written with selects (no divergent branches); the hit record is
+    // evaluated once, after the loop.
     const uint32_t* occ = s.occ + (size_t)brick * 16u;
     uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
     uint32_t word = occ[lin >> 5];
+    bool hit = false;
     for (int it = 0;; ++it) {
         if (COUNT) cnt.voxels += 1;
-        int n0 = cur[0], n1 = cur[1], n2 = cur[2];
-        bool stop;
-        if (tNext[0] < tNext[1] && tNext[0] < tNext[2]) {
-            stop = tNext[0] > btmax;
-            n0 += step[0];
-            tNext[0] += tDelta[0];
-        } else if (tNext[1] < tNext[2]) {
-            stop = tNext[1] > btmax;
-            n1 += step[1];
-            tNext[1] += tDelta[1];
-        } else {
-            stop = tNext[2] > btmax;
-            n2 += step[2];
-            tNext[2] += tDelta[2];
-        }
-        const bool more = !stop && it + 1 < 24 && (uint32_t)n0 <= 7u && (uint32_t)n1 <= 7u && (uint32_t)n2 <= 7u;
+        // bitwise & on the conditions: no short-circuit branches
+        const bool ax = (tNext[0] < tNext[1]) & (tNext[0] < tNext[2]);
+        const bool ay = !ax & (tNext[1] < tNext[2]);
+        const bool az = !ax & !ay;
+        const float tsel = ax ? tNext[0] : (ay ? tNext[1] : tNext[2]);
+        const int n0 = cur[0] + (ax ? step[0] : 0);
+        const int n1 = cur[1] + (ay ? step[1] : 0);
+        const int n2 = cur[2] + (az ? step[2] : 0);
+        const bool more = !(tsel > btmax) & (it + 1 < 24) & ((uint32_t)(n0 | n1 | n2) <= 7u);
         const uint32_t nlin = (uint32_t)((n0 & 7) + 8 * (n1 & 7) + 64 * (n2 & 7));
         const uint32_t nword = occ[nlin >> 5];
-        if ((word >> (lin & 31u)) & 1u) {
-            float tnh[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                if (__builtin_fabsf(dl[i]) < 1e-8f) {
-                    tnh[i] = -__builtin_inff();
-                } else {
-                    const float vmin = bmin[i] + (float)cur[i];
-                    const float vmax = vmin + 1.0f;
-                    const float t0 = (vmin - ol[i]) / dl[i];
-                    const float t1 = (vmax - ol[i]) / dl[i];
-                    tnh[i] = mn(t0, t1);
-                }
-            }
-            float tHit = mx(mx(tnh[0], tnh[1]), tnh[2]);
-            tHit = mx(tHit, tstart);
-            // sign bits computed for all axes and selected as values: an indexed
-            // read of dl[] here would make the ray state addressable (scratch)
-            const uint32_t sx = dl[0] > 0.0f ? 0u : 1u, sy = dl[1] > 0.0f ? 0u : 1u, sz = dl[2] > 0.0f ? 0u : 1u;
-            uint32_t axis, signpos;
-            if (tnh[0] >= tnh[1] && tnh[0] >= tnh[2]) {
-                axis = 0u;
-                signpos = sx;
-            } else if (tnh[1] >= tnh[2]) {
-                axis = 1u;
-                signpos = sy;
-            } else {
-                axis = 2u;
-                signpos = sz;
-            }
-            h.flags = 1u | (axis << 1) | (signpos << 3);
-            h.t = tHit;
-            h.brick = brick;
-            h.voxel = lin;
-            if (s.bc1) bc1_decode(s.bc1[(size_t)brick * 32u + (lin >> 4)], lin & 15u, h.rgb);
-            else rgba8_decode(s.brick_data[(size_t)brick * 512u + lin], h.rgb);
-            return true;
+        const bool occupied = (word >> (lin & 31u)) & 1u;
+        if (occupied | !more) {
+            hit = occupied;
+            break;
         }
-        if (!more) return false;
+        const float s0 = tNext[0] + tDelta[0], s1 = tNext[1] + tDelta[1], s2 = tNext[2] + tDelta[2];
+        tNext[0] = ax ? s0 : tNext[0];
+        tNext[1] = ay ? s1 : tNext[1];
+        tNext[2] = az ? s2 : tNext[2];
         cur[0] = n0;
         cur[1] = n1;
         cur[2] = n2;
         lin = nlin;
         word = nword;
     }
+    if (!hit) return false;
+
+    float tnh[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (__builtin_fabsf(dl[i]) < 1e-8f) {
+            tnh[i] = -__builtin_inff();
+        } else {
+            const float vmin = bmin[i] + (float)cur[i];
+            const float vmax = vmin + 1.0f;
+            const float t0 = (vmin - ol[i]) / dl[i];
+            const float t1 = (vmax - ol[i]) / dl[i];
+            tnh[i] = mn(t0, t1);
+        }
+    }
+    float tHit = mx(mx(tnh[0], tnh[1]), tnh[2]);
+    tHit = mx(tHit, tstart);
+    // sign bits computed for all axes and selected as values: an indexed
+    // read of dl[] here would make the ray state addressable (scratch)
+    const uint32_t sx = dl[0] > 0.0f ? 0u : 1u, sy = dl[1] > 0.0f ? 0u : 1u, sz = dl[2] > 0.0f ? 0u : 1u;
+    uint32_t axis, signpos;
+    if (tnh[0] >= tnh[1] && tnh[0] >= tnh[2]) {
+        axis = 0u;
+        signpos = sx;
+    } else if (tnh[1] >= tnh[2]) {
+        axis = 1u;
+        signpos = sy;
+    } else {
+        axis = 2u;
+        signpos = sz;
+    }
+    h.flags = 1u | (axis << 1) | (signpos << 3);
+    h.t = tHit;
+    h.brick = brick;
+    h.voxel = lin;
+    if (s.bc1) bc1_decode(s.bc1[(size_t)brick * 32u + (lin >> 4)], lin & 15u, h.rgb);
+    else rgba8_decode(s.brick_data[(size_t)brick * 512u + lin], h.rgb);
+    return true;
 }
 
 // Descriptor fetch: a prefix of the descriptor array (the top levels of a
@@ -438,20 +442,18 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
     const int octm = r.mask & 7;
     const int local = r.idx ^ ((~octm) & 7);
 
-    bool process = false;
-    float tv_max = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    if (((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS)) {
-        const float tx = r.pos[0] * r.tc[0] - r.tb[0];
-        const float ty = r.pos[1] * r.tc[1] - r.tb[1];
-        const float tz = r.pos[2] * r.tc[2] - r.tb[2];
-        const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
-        tv_max = mn(r.t_max, tcm);
-        const float half = r.scale_exp2 * 0.5f;
-        cx = half * r.tc[0] + tx;
-        cy = half * r.tc[1] + ty;
-        cz = half * r.tc[2] + tz;
-        process = r.t_min <= tv_max + T_EPS;
-    }
+    // The child's exit times serve both the PUSH test and ADVANCE (neither pos,
+    // mask nor t_max changes in between), so they are evaluated once, unconditionally.
+    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
+    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
+    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
+    const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
+    const float tv_max = mn(r.t_max, tcm);
+    const float half = r.scale_exp2 * 0.5f;
+    const float cx = half * r.tc[0] + tx;
+    const float cy = half * r.tc[1] + ty;
+    const float cz = half * r.tc[2] + tz;
+    const bool process = ((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS) && r.t_min <= tv_max + T_EPS;
     if (process) {
         if ((leaf >> local) & 1u) {
             if (COUNT) cnt.leaves += 1;
@@ -481,7 +483,6 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
             r.parent = cp + popc8(valid & ~leaf & below);
             int idx = 0;
             r.scale -= 1;
-            const float half = r.scale_exp2 * 0.5f;
             r.scale_exp2 = half;
             if (cx > r.t_min) { idx ^= 1; r.pos[0] += r.scale_exp2; }
             if (cy > r.t_min) { idx ^= 2; r.pos[1] += r.scale_exp2; }
@@ -494,17 +495,14 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
     }
 
     // ADVANCE
-    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
-    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
-    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
     const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
-    float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
-    if (tcm >= 1e10f) tcm = mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f);
+    const float tca =
+        tcm >= 1e10f ? mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f) : tcm;
     int step_mask = 0;
-    if (canx && tx <= tcm) { step_mask ^= 1; r.pos[0] -= r.scale_exp2; }
-    if (cany && ty <= tcm) { step_mask ^= 2; r.pos[1] -= r.scale_exp2; }
-    if (canz && tz <= tcm) { step_mask ^= 4; r.pos[2] -= r.scale_exp2; }
-    r.t_min = mx(tcm, 0.0f);
+    if (canx && tx <= tca) { step_mask ^= 1; r.pos[0] -= r.scale_exp2; }
+    if (cany && ty <= tca) { step_mask ^= 2; r.pos[1] -= r.scale_exp2; }
+    if (canz && tz <= tca) { step_mask ^= 4; r.pos[2] -= r.scale_exp2; }
+    r.t_min = mx(tca, 0.0f);
     r.idx ^= step_mask;
     r.iter = iter + 1;
 
