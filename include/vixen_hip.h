@@ -206,7 +206,8 @@ vx_status vx_trace_counters(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_
                             uint64_t* counters, vx_stream stream);
 
 /* Levels of the breadth-first descriptor array (0-4) that vx_trace/vx_render
- * stage in LDS per workgroup (a prefix of the array, at most 64 KiB). */
+ * stage in LDS per workgroup (a prefix of the array, at most 64 KiB).
+ * Default 0: no staging, descriptors are read from global memory. */
 vx_status vx_scene_set_lds_depth(vx_scene* scene, uint32_t depth);
 
 /* ------------------------------------------------------------------------ */

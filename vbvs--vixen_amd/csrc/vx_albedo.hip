@@ -96,15 +96,9 @@ __device__ __forceinline__ uint32_t trace_pixel(const SceneView& s, const AlbCam
     vx_hit hr;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     shade = 0.0f;
-    if (!trace_ray<false>(s, c.pos, d, 0.0f, 3.0e38f, hr, cnt)) return kNoVoxel;
+    if (!trace_ray<false, false>(s, c.pos, d, 0.0f, 3.0e38f, hr, cnt)) return kNoVoxel;
     shade = face_shade(hr.flags, c.light);
     return voxel_row(s, base, hr.brick, hr.voxel);
-}
-
-__device__ __forceinline__ void stage_alb(const SceneView& s)
-{
-    for (uint32_t i = threadIdx.x; i < s.lds_count; i += blockDim.x) vx_lds_nodes[i] = s.nodes[i];
-    __syncthreads();
 }
 
 // Pixel of lane threadIdx.x in 16x16 tile blockIdx.x (row-major tiles, 8x8 per wave).
@@ -122,7 +116,6 @@ __global__ void __launch_bounds__(kAlbBlock)
                      const float* __restrict__ albedo, float* __restrict__ image, uint32_t* __restrict__ hit_row,
                      float* __restrict__ hit_shade)
 {
-    stage_alb(s);
     uint32_t px, py;
     if (!tile_pixel(w, h, px, py)) return;
     float shade;
@@ -167,7 +160,6 @@ __global__ void __launch_bounds__(kAlbBlock)
                      const float* __restrict__ hit_shade, float* __restrict__ grad, uint32_t* __restrict__ keys,
                      Rgb* __restrict__ vals)
 {
-    if (RECOMPUTE) stage_alb(s);
     uint32_t px, py;
     const bool inside = tile_pixel(w, h, px, py);
     const uint64_t p = (uint64_t)py * w + px;
@@ -329,15 +321,15 @@ static AlbCam make_alb_cam(const vx_camera* cam, const float light[3])
     return c;
 }
 
+// The albedo kernels read descriptors from global memory (no staged prefix);
+// their LDS holds the traversal stacks only.
 static SceneView alb_view(const vx_scene* s, uint32_t& lds)
 {
     SceneView v = s->view;
     v.lds_off = 0;
-    v.lds_count = staged_nodes(s, s->lds_depth);
-    const uint32_t cap = (kMaxLdsBytes - stack_bytes(kAlbBlock)) / 8u;
-    if (v.lds_count > cap) v.lds_count = cap;
-    v.stack_base = v.lds_count * 2u;
-    lds = v.lds_count * 8u + stack_bytes(kAlbBlock);
+    v.lds_count = 0;
+    v.stack_base = 0;
+    lds = stack_bytes(kAlbBlock);
     return v;
 }
 

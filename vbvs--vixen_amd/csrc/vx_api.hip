@@ -6,7 +6,10 @@
 // band of tiles/ray chunks (the hardware deals workgroup i to XCD i % 8), which
 // keeps neighbouring rays -- and the descriptors they share -- in one XCD's L2.
 // A prefix of the descriptor array (the top breadth-first levels) can be staged
-// in LDS per workgroup; vx_scene_set_lds_depth picks how many levels.
+// in LDS per workgroup; vx_scene_set_lds_depth picks how many levels. Depth 0
+// (the default) launches kernels compiled without staging: their descriptor
+// fetch is a plain global load, which measures faster on gfx950 than the flat
+// load a runtime LDS-or-global choice compiles to (the top levels stay in L2).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -64,8 +67,10 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t per_xcd)
 }
 
 // Every lane of the workgroup copies part of the descriptor prefix into LDS.
+template <bool STAGED>
 __device__ __forceinline__ void stage_prefix(const SceneView& s)
 {
+    if (!STAGED) return;
     for (uint32_t i = threadIdx.x; i < s.lds_count; i += blockDim.x) vx_lds_nodes[i] = s.nodes[i];
     __syncthreads();
 }
@@ -78,19 +83,19 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
     if ((threadIdx.x & 63u) == 0u) atomicAdd(dst, x);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool STAGED>
 __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __restrict__ rays, uint64_t n,
                                                   uint32_t per_xcd, vx_hit* __restrict__ hits,
                                                   unsigned long long* __restrict__ counters)
 {
-    stage_prefix(s);
+    stage_prefix<STAGED>(s);
     const uint64_t i = (uint64_t)xcd_chunk(per_xcd) * kBlock + threadIdx.x;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     if (i < n) {
         float o[3], d[3], tmin, tmax;
         load_ray(rays, i, o, d, tmin, tmax);
         vx_hit h;
-        trace_ray<COUNT>(s, o, d, tmin, tmax, h, cnt);
+        trace_ray<COUNT, STAGED>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
     if (COUNT) {
@@ -127,11 +132,12 @@ __device__ __forceinline__ uint32_t to_u8(float x)
 }
 
 // 16x16 pixel tile per workgroup; each wave covers an 8x8 sub-tile.
+template <bool STAGED>
 __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uint32_t w, uint32_t h, float tmax,
                                                    uint32_t tiles_x, uint32_t ntiles, uint32_t per_xcd,
                                                    vx_hit* __restrict__ hits, uint32_t* __restrict__ rgba)
 {
-    stage_prefix(s);
+    stage_prefix<STAGED>(s);
     const uint32_t tile = xcd_chunk(per_xcd);
     if (tile >= ntiles) return;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -142,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uin
     camera_dir(c, px, py, w, h, d, v);
     vx_hit hr;
     TraceCounts cnt{0u, 0u, 0u, 0u};
-    const bool hit = trace_ray<false>(s, c.pos, d, 0.0f, tmax, hr, cnt);
+    const bool hit = trace_ray<false, STAGED>(s, c.pos, d, 0.0f, tmax, hr, cnt);
     const uint64_t i = (uint64_t)py * w + px;
     if (hits) store_hit(hits, i, hr);
     if (rgba) {
@@ -393,7 +399,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.lds_off = 0;
     v.lds_count = 0;
     v.stack_base = 0;
-    s->lds_depth = 2;
+    s->lds_depth = 0;
     *out = s;
     return VX_OK;
 }
@@ -422,12 +428,15 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
     const SceneView v = launch_view(s);
     const size_t lds = launch_lds(v);
+    const bool staged = v.lds_count > 0;
     if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
-        k_trace<true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits,
-                                                reinterpret_cast<unsigned long long*>(counters));
+        auto* c = reinterpret_cast<unsigned long long*>(counters);
+        if (staged) k_trace<true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
+        else k_trace<true, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
     } else {
-        k_trace<false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        if (staged) k_trace<false, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        else k_trace<false, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
     }
     VX_HIP(hipGetLastError());
     return VX_OK;
@@ -541,8 +550,12 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const vx::SceneView v = vx::launch_view(s);
-    vx::k_render<<<dim3(per * vx::kXcds), dim3(vx::kBlock), vx::launch_lds(v), st>>>(
-        v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
+    const dim3 grid(per * vx::kXcds), block(vx::kBlock);
+    if (v.lds_count > 0)
+        vx::k_render<true><<<grid, block, vx::launch_lds(v), st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
+    else
+        vx::k_render<false><<<grid, block, vx::launch_lds(v), st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits,
+                                                                    rgba8);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
     return VX_OK;
 }

@@ -54,18 +54,21 @@ __device__ __forceinline__ uint32_t find_scene(const BatchSceneInfo* __restrict_
 
 __global__ void k_reset_queue(unsigned int* q) { *q = 0u; }
 
+template <bool STAGED>
 __global__ void __launch_bounds__(kMaxGroup)
     k_trace_persistent(const BatchSceneInfo* __restrict__ scenes, uint32_t n_scenes, uint64_t n_chunks,
                        uint32_t chunk, uint32_t refill_min, uint32_t stack_base, const vx_ray* __restrict__ rays,
                        vx_hit* __restrict__ hits, unsigned int* __restrict__ queue)
 {
     // stage every scene's descriptor prefix (lds_off/lds_count set per launch)
-    for (uint32_t s = 0; s < n_scenes; ++s) {
-        const uint2* src = scenes[s].view.nodes;
-        const uint32_t off = scenes[s].view.lds_off, cnt = scenes[s].view.lds_count;
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) vx_lds_nodes[off + i] = src[i];
+    if (STAGED) {
+        for (uint32_t s = 0; s < n_scenes; ++s) {
+            const uint2* src = scenes[s].view.nodes;
+            const uint32_t off = scenes[s].view.lds_off, cnt = scenes[s].view.lds_count;
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) vx_lds_nodes[off + i] = src[i];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t full = ~0ull;
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(kMaxGroup)
             if (drained && pending == kNoChunk && cur == end) break;
             continue;
         }
-        if (active && step_ray<false>(sv, r, h, cnt)) {
+        if (active && step_ray<false, STAGED>(sv, r, h, cnt)) {
             store_hit(hits, ray_id, h);
             active = false;
         }
@@ -166,7 +169,7 @@ struct vx_graph {
 
 namespace vx {
 
-static vx_tuning default_tuning() { return vx_tuning{4, 128, 2, 32}; }
+static vx_tuning default_tuning() { return vx_tuning{4, 128, 0, 32}; }
 
 static vx_tuning sanitize(const vx_tuning* t, const vx_batch* b)
 {
@@ -210,11 +213,12 @@ static bool same_table(const std::vector<BatchSceneInfo>& a, const std::vector<B
     return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * sizeof(BatchSceneInfo)) == 0;
 }
 
-static vx_status grid_size(const vx_batch* b, const vx_tuning& t, uint32_t lds_bytes, unsigned& groups)
+static vx_status grid_size(const vx_batch* b, const vx_tuning& t, bool staged, uint32_t lds_bytes, unsigned& groups)
 {
     const int block = static_cast<int>(t.waves_per_group * 64);
     int per_cu = 0;
-    VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent, block, lds_bytes));
+    if (staged) VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<true>, block, lds_bytes));
+    else VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<false>, block, lds_bytes));
     if (per_cu < 1) return fail(VX_ERR_UNSUPPORTED, "persistent kernel does not fit one workgroup per CU");
     groups = static_cast<unsigned>(per_cu * b->ctx->cu_count);
     return VX_OK;
@@ -224,11 +228,15 @@ static vx_status enqueue(vx_batch* b, BatchSceneInfo* d_table, unsigned int* d_q
                          const vx_tuning& t, uint32_t lds_bytes, const vx_ray* rays, vx_hit* hits, hipStream_t st)
 {
     unsigned groups = 0;
+    const bool staged = lds_bytes > 0; // some scene has a staged prefix
     const uint32_t total_lds = lds_bytes + stack_bytes(t.waves_per_group * 64);
-    if (vx_status s = grid_size(b, t, total_lds, groups); s != VX_OK) return s;
+    if (vx_status s = grid_size(b, t, staged, total_lds, groups); s != VX_OK) return s;
     k_reset_queue<<<1, 1, 0, st>>>(d_queue);
-    if (n_chunks)
-        k_trace_persistent<<<groups, t.waves_per_group * 64, total_lds, st>>>(
+    if (n_chunks && staged)
+        k_trace_persistent<true><<<groups, t.waves_per_group * 64, total_lds, st>>>(
+            d_table, b->n, n_chunks, t.chunk, t.refill_min, lds_bytes / 4u, rays, hits, d_queue);
+    else if (n_chunks)
+        k_trace_persistent<false><<<groups, t.waves_per_group * 64, total_lds, st>>>(
             d_table, b->n, n_chunks, t.chunk, t.refill_min, lds_bytes / 4u, rays, hits, d_queue);
     VX_HIP(hipGetLastError());
     return VX_OK;

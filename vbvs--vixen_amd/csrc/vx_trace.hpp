@@ -313,9 +313,12 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
 // word (level * blockDim.x + threadIdx.x) of each half).
 extern __shared__ uint2 vx_lds_nodes[];
 
+// STAGED is a compile-time choice: without staging the fetch is a plain global
+// load (a runtime LDS-or-global choice compiles to a slower flat load).
+template <bool STAGED>
 __device__ __forceinline__ uint2 fetch_node(const SceneView& s, uint32_t i)
 {
-    if (i < s.lds_count) return vx_lds_nodes[s.lds_off + i];
+    if (STAGED && i < s.lds_count) return vx_lds_nodes[s.lds_off + i];
     return s.nodes[i];
 }
 
@@ -423,7 +426,7 @@ __device__ __forceinline__ float corrected_tc_max_f(float tx, float ty, float tz
 
 // One iteration of the ESVO loop (VoxelRayMarch_Compressed.comp:396-478).
 // Returns true when the ray is finished; h then holds its record.
-template <bool COUNT>
+template <bool COUNT, bool STAGED>
 __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
 {
     const int iter = r.iter;
@@ -436,7 +439,7 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
         store_miss(h, (uint32_t)(iter + 1));
         return true;
     }
-    const uint2 desc = fetch_node(s, r.parent);
+    const uint2 desc = fetch_node<STAGED>(s, r.parent);
     const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
     const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
     const int octm = r.mask & 7;
@@ -460,7 +463,7 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
             const uint32_t below = (1u << local) - 1u;
             const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
             if (lidx < s.node_count) {
-                const uint32_t brick = fetch_node(s, lidx).y & 0xFFFFFFu;
+                const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
                 if (brick != 0xFFFFFFu && brick < s.brick_count) {
                     int bc[3];
 #pragma unroll
@@ -546,13 +549,13 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
 }
 
 // Full traversal of one ray. Writes the hit record; returns true on a hit.
-template <bool COUNT>
+template <bool COUNT, bool STAGED>
 __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
                                           float tmax, vx_hit& h, TraceCounts& cnt)
 {
     RayState r;
     if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
-    while (!step_ray<COUNT>(s, r, h, cnt)) {
+    while (!step_ray<COUNT, STAGED>(s, r, h, cnt)) {
     }
     return (h.flags & 1u) != 0u;
 }
