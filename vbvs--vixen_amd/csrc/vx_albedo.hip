@@ -96,7 +96,7 @@ __device__ __forceinline__ uint32_t trace_pixel(const SceneView& s, const AlbCam
     vx_hit hr;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     shade = 0.0f;
-    if (!trace_ray<false, false>(s, c.pos, d, 0.0f, 3.0e38f, hr, cnt)) return kNoVoxel;
+    if (!trace_ray<false, false, false>(s, c.pos, d, 0.0f, 3.0e38f, hr, cnt)) return kNoVoxel;
     shade = face_shade(hr.flags, c.light);
     return voxel_row(s, base, hr.brick, hr.voxel);
 }
@@ -328,8 +328,7 @@ static SceneView alb_view(const vx_scene* s, uint32_t& lds)
     SceneView v = s->view;
     v.lds_off = 0;
     v.lds_count = 0;
-    v.stack_base = 0;
-    lds = stack_bytes(kAlbBlock);
+    lds = lay_out_lanes(v, 0, kAlbBlock);
     return v;
 }
 

@@ -12,6 +12,7 @@
 // load a runtime LDS-or-global choice compiles to (the top levels stay in L2).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
         float o[3], d[3], tmin, tmax;
         load_ray(rays, i, o, d, tmin, tmax);
         vx_hit h;
-        trace_ray<COUNT, STAGED>(s, o, d, tmin, tmax, h, cnt);
+        trace_ray<COUNT, STAGED, true>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
     if (COUNT) {
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uin
     camera_dir(c, px, py, w, h, d, v);
     vx_hit hr;
     TraceCounts cnt{0u, 0u, 0u, 0u};
-    const bool hit = trace_ray<false, STAGED>(s, c.pos, d, 0.0f, tmax, hr, cnt);
+    const bool hit = trace_ray<false, STAGED, false>(s, c.pos, d, 0.0f, tmax, hr, cnt);
     const uint64_t i = (uint64_t)py * w + px;
     if (hits) store_hit(hits, i, hr);
     if (rgba) {
@@ -327,7 +328,7 @@ static vx_status scene_validate_impl(const vx_scene_desc* d, uint32_t level_pref
     const int levels = ilog2(B);
     if (c.brick_size != 8 || c.brick_depth_levels != 3 || c.user_max_levels != levels + 3 || c.esvo_max_scale != 22)
         return fail(VX_ERR_INVALID_SCENE, "config must describe 8^3 bricks under log2(bricks_per_axis) levels, esvo scale 22");
-    if (levels - 1 > VX_STACK) return fail(VX_ERR_UNSUPPORTED, "tree deeper than the register stack");
+    if (levels - 1 > VX_STACK) return fail(VX_ERR_UNSUPPORTED, "tree deeper than the traversal stack");
     const float sx = c.grid_max[0] - c.grid_min[0], sy = c.grid_max[1] - c.grid_min[1], sz = c.grid_max[2] - c.grid_min[2];
     if (!(sx > 0.0f) || sx != sy || sx != sz || !std::isfinite(sx))
         return fail(VX_ERR_INVALID_SCENE, "grid bounds must be a non-empty cube");
@@ -398,24 +399,25 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.min_scale = 22 - c.user_max_levels + 1;
     v.lds_off = 0;
     v.lds_count = 0;
+    v.stack_depth = static_cast<uint32_t>(std::max(1, ilog2(B) - 1));
     v.stack_base = 0;
+    v.aux_base = 0;
     s->lds_depth = 0;
     *out = s;
     return VX_OK;
 }
 
-static SceneView launch_view(const vx_scene* s)
+// The scene's view for a plain launch of kBlock lanes; lds receives the dynamic LDS bytes.
+static SceneView launch_view(const vx_scene* s, uint32_t& lds)
 {
     SceneView v = s->view;
     v.lds_off = 0;
     v.lds_count = staged_nodes(s, s->lds_depth);
-    const uint32_t cap = (kMaxLdsBytes - stack_bytes(kBlock)) / 8u;
+    const uint32_t cap = (kMaxLdsBytes - kBlock * lane_lds_bytes(v.stack_depth)) / 8u;
     if (v.lds_count > cap) v.lds_count = cap;
-    v.stack_base = v.lds_count * 2u;
+    lds = lay_out_lanes(v, v.lds_count, kBlock);
     return v;
 }
-
-static size_t launch_lds(const SceneView& v) { return v.lds_count * sizeof(uint2) + stack_bytes(kBlock); }
 
 static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
                               hipStream_t st)
@@ -426,8 +428,8 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     const uint64_t per = (chunks + kXcds - 1) / kXcds;
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
-    const SceneView v = launch_view(s);
-    const size_t lds = launch_lds(v);
+    uint32_t lds = 0;
+    const SceneView v = launch_view(s, lds);
     const bool staged = v.lds_count > 0;
     if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
@@ -549,13 +551,13 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const vx::SceneView v = vx::launch_view(s);
+    uint32_t lds = 0;
+    const vx::SceneView v = vx::launch_view(s, lds);
     const dim3 grid(per * vx::kXcds), block(vx::kBlock);
     if (v.lds_count > 0)
-        vx::k_render<true><<<grid, block, vx::launch_lds(v), st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
+        vx::k_render<true><<<grid, block, lds, st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
     else
-        vx::k_render<false><<<grid, block, vx::launch_lds(v), st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits,
-                                                                    rgba8);
+        vx::k_render<false><<<grid, block, lds, st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
     return VX_OK;
 }

@@ -57,7 +57,7 @@ __global__ void k_reset_queue(unsigned int* q) { *q = 0u; }
 template <bool STAGED>
 __global__ void __launch_bounds__(kMaxGroup)
     k_trace_persistent(const BatchSceneInfo* __restrict__ scenes, uint32_t n_scenes, uint64_t n_chunks,
-                       uint32_t chunk, uint32_t refill_min, uint32_t stack_base, const vx_ray* __restrict__ rays,
+                       uint32_t chunk, uint32_t refill_min, const vx_ray* __restrict__ rays,
                        vx_hit* __restrict__ hits, unsigned int* __restrict__ queue)
 {
     // stage every scene's descriptor prefix (lds_off/lds_count set per launch)
@@ -75,8 +75,7 @@ __global__ void __launch_bounds__(kMaxGroup)
     uint64_t cur = 0, end = 0, pending = kNoChunk;
     int scene = -1;
     bool drained = false;
-    SceneView sv = scenes[0].view;
-    sv.stack_base = stack_base;
+    SceneView sv = scenes[0].view; // every view carries the launch's per-lane LDS layout
 
     bool active = false;
     uint64_t ray_id = 0;
@@ -111,7 +110,6 @@ __global__ void __launch_bounds__(kMaxGroup)
                     if (cs != scene) {
                         scene = cs;
                         sv = scenes[cs].view;
-                        sv.stack_base = stack_base;
                     }
                     const uint64_t rb = scenes[cs].ray_begin, rc = scenes[cs].ray_count;
                     cur = rb + (c - scenes[cs].chunk_begin) * chunk;
@@ -181,13 +179,18 @@ static vx_tuning sanitize(const vx_tuning* t, const vx_batch* b)
     return r;
 }
 
-// Builds the per-launch scene table; returns the LDS bytes it needs.
+// Builds the per-launch scene table; returns the kernel's LDS bytes (staged
+// descriptors and per-lane columns) and whether any scene has a staged prefix.
 static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tuning& t,
-                            std::vector<BatchSceneInfo>& tab, uint64_t& n_chunks, uint32_t& lds_bytes)
+                            std::vector<BatchSceneInfo>& tab, uint64_t& n_chunks, uint32_t& lds_bytes, bool& staged)
 {
     tab.resize(b->n);
     uint64_t chunk_acc = 0;
     uint32_t lds_acc = 0;
+    // one per-lane layout for the launch: the deepest scene's stack
+    uint32_t depth = 1;
+    for (uint32_t s = 0; s < b->n; ++s) depth = std::max(depth, b->views[s].stack_depth);
+    const uint32_t lanes = t.waves_per_group * 64, lane_bytes = lanes * lane_lds_bytes(depth);
     for (uint32_t s = 0; s < b->n; ++s) {
         if (off[s + 1] < off[s]) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: ray_offsets must be non-decreasing");
         BatchSceneInfo& e = tab[s];
@@ -198,13 +201,18 @@ static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tun
         chunk_acc += (e.ray_count + t.chunk - 1) / t.chunk;
         uint32_t want = 0;
         if (t.lds_depth) want = b->level_prefix[4 * s + std::min<uint32_t>(t.lds_depth, 4) - 1];
-        if ((lds_acc + want) * 8u + stack_bytes(t.waves_per_group * 64) > kMaxLdsBytes) want = 0;
+        if ((lds_acc + want) * 8u + lane_bytes > kMaxLdsBytes) want = 0;
         e.view.lds_off = lds_acc;
         e.view.lds_count = want;
         lds_acc += want;
     }
+    for (BatchSceneInfo& e : tab) {
+        e.view.stack_depth = depth;
+        lay_out_lanes(e.view, lds_acc, lanes);
+    }
     n_chunks = chunk_acc;
-    lds_bytes = lds_acc * 8u;
+    lds_bytes = lds_acc * 8u + lane_bytes;
+    staged = lds_acc > 0;
     return VX_OK;
 }
 
@@ -225,19 +233,18 @@ static vx_status grid_size(const vx_batch* b, const vx_tuning& t, bool staged, u
 }
 
 static vx_status enqueue(vx_batch* b, BatchSceneInfo* d_table, unsigned int* d_queue, uint64_t n_chunks,
-                         const vx_tuning& t, uint32_t lds_bytes, const vx_ray* rays, vx_hit* hits, hipStream_t st)
+                         const vx_tuning& t, bool staged, uint32_t lds_bytes, const vx_ray* rays, vx_hit* hits,
+                         hipStream_t st)
 {
     unsigned groups = 0;
-    const bool staged = lds_bytes > 0; // some scene has a staged prefix
-    const uint32_t total_lds = lds_bytes + stack_bytes(t.waves_per_group * 64);
-    if (vx_status s = grid_size(b, t, staged, total_lds, groups); s != VX_OK) return s;
+    if (vx_status s = grid_size(b, t, staged, lds_bytes, groups); s != VX_OK) return s;
     k_reset_queue<<<1, 1, 0, st>>>(d_queue);
     if (n_chunks && staged)
-        k_trace_persistent<true><<<groups, t.waves_per_group * 64, total_lds, st>>>(
-            d_table, b->n, n_chunks, t.chunk, t.refill_min, lds_bytes / 4u, rays, hits, d_queue);
+        k_trace_persistent<true><<<groups, t.waves_per_group * 64, lds_bytes, st>>>(
+            d_table, b->n, n_chunks, t.chunk, t.refill_min, rays, hits, d_queue);
     else if (n_chunks)
-        k_trace_persistent<false><<<groups, t.waves_per_group * 64, total_lds, st>>>(
-            d_table, b->n, n_chunks, t.chunk, t.refill_min, lds_bytes / 4u, rays, hits, d_queue);
+        k_trace_persistent<false><<<groups, t.waves_per_group * 64, lds_bytes, st>>>(
+            d_table, b->n, n_chunks, t.chunk, t.refill_min, rays, hits, d_queue);
     VX_HIP(hipGetLastError());
     return VX_OK;
 }
@@ -251,7 +258,8 @@ static vx_status trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* 
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
     uint32_t lds_bytes = 0;
-    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes); s != VX_OK) return s;
+    bool staged = false;
+    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes, staged); s != VX_OK) return s;
     if (n_chunks > 0xFFFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: more than 2^32 chunks");
     if (!same_table(tab, b->h_table)) {
         // the device table may still be read by an earlier launch on any stream
@@ -259,7 +267,7 @@ static vx_status trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* 
         VX_HIP(hipMemcpy(b->d_table, tab.data(), tab.size() * sizeof(BatchSceneInfo), hipMemcpyHostToDevice));
         b->h_table = tab;
     }
-    return enqueue(b, b->d_table, b->d_queue, n_chunks, t, lds_bytes, rays, hits, st);
+    return enqueue(b, b->d_table, b->d_queue, n_chunks, t, staged, lds_bytes, rays, hits, st);
 }
 
 static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, vx_batch** out)
@@ -341,8 +349,9 @@ static vx_status autotune(vx_batch* b, const vx_ray* rays, const uint64_t* off, 
                 std::vector<BatchSceneInfo> tab;
                 uint64_t n_chunks = 0;
                 uint32_t lds_bytes = 0;
-                if ((status = make_table(b, off, t, tab, n_chunks, lds_bytes)) != VX_OK) break;
-                if (depth > 0 && lds_bytes == 0) continue; // nothing staged: same as depth 0
+                bool staged = false;
+                if ((status = make_table(b, off, t, tab, n_chunks, lds_bytes, staged)) != VX_OK) break;
+                if (depth > 0 && !staged) continue; // nothing staged: same as depth 0
                 float ms = 1e30f;
                 for (int rep = 0; rep < 3 && status == VX_OK; ++rep) {
                     if (hipEventRecord(e0, st) != hipSuccess) status = fail(VX_ERR_HIP, "event record failed");
@@ -375,7 +384,8 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
     uint32_t lds_bytes = 0;
-    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes); s != VX_OK) return s;
+    bool staged = false;
+    if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes, staged); s != VX_OK) return s;
     auto* g = new (std::nothrow) vx_graph{};
     if (!g) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: out of memory");
     auto cleanup = [&](vx_status st) {
@@ -398,7 +408,7 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         st = fail(VX_ERR_HIP, "vx_batch_graph_create: begin capture failed");
     } else {
-        st = enqueue(b, g->d_table, g->d_queue, n_chunks, t, lds_bytes, rays, hits, cs);
+        st = enqueue(b, g->d_table, g->d_queue, n_chunks, t, staged, lds_bytes, rays, hits, cs);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(cs, &graph);
         g->graph = graph;

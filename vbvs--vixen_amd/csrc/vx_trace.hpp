@@ -14,11 +14,18 @@
 //   BC1 decode                : libraries/VoxelData/src/Compression/DXT1Compressor.cpp:14-36
 //
 // Layout choices that are this backend's own:
-//   - the traversal stack only holds the scales a well-formed tree can reach
-//     (22 down to one above the brick level, at most VX_STACK entries) and lives
-//     in LDS, one lane-strided column per lane (bank-conflict free, no VGPRs,
-//     no scratch); every entry a ray reads was written earlier on its own path,
-//     so the stack needs no initialisation;
+//   - the traversal stack only holds the scales the scene's tree can reach
+//     (22 down to one above the brick level: log2(bricks per axis) - 1 entries,
+//     at most VX_STACK) and lives in LDS, one lane-strided column per lane
+//     (bank-conflict free, no VGPRs, no scratch); every entry a ray reads was
+//     written earlier on its own path, so the stack needs no initialisation;
+//   - the ray's brick-DDA inputs (local origin and direction, reciprocal
+//     direction, traced segment: 11 words) and a pending leaf (3 words) are
+//     needed only at brick visits, so they too live in lane-strided LDS
+//     columns instead of VGPRs across the ESVO loop (more waves per SIMD);
+//   - for incoherent rays, brick visits are postponed until most live lanes of
+//     the wave wait at one, so the wave runs the DDA loop once for all of them
+//     (trace_ray);
 //   - brick occupancy is read from a 64-byte-per-brick bit mask derived from
 //     brickData at upload (bit lin of the brick's 16 words), so a brick's DDA
 //     touches one cache line instead of 2 KiB; the hit voxel's brickData word
@@ -58,6 +65,8 @@ struct SceneView {
     uint32_t lds_off;    // first LDS slot of this scene's staged descriptors
     uint32_t lds_count;  // descriptors [0, lds_count) are read from LDS
     uint32_t stack_base; // first LDS word of the per-lane traversal stacks
+    uint32_t stack_depth; // stack entries per lane (levels above the bricks - 1, >= 1)
+    uint32_t aux_base;   // first LDS word of the per-lane brick-DDA inputs
 };
 
 struct TraceCounts {
@@ -138,7 +147,7 @@ __device__ __forceinline__ void rgba8_decode(uint32_t w, float rgb[3])
 // or an invalid ray.
 __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
                                           float tmax, float ol[3], float dl[3], float du[3], float& tstart,
-                                          float& texit)
+                                          float& texit, float invd[3])
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -146,7 +155,6 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
     const float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     if (!(len2 > 0.0f) || !__builtin_isfinite(len2)) return false;
     const float inv = 1.0f / __builtin_sqrtf(len2);
-    float invd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         du[i] = d[i] * inv;
@@ -180,19 +188,21 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
 }
 
 // Brick DDA in local voxel space. Returns true on a hit and fills the hit record.
+// rinv[i] is the ray's brick reciprocal, 1/dl[i] (+-1e8 when |dl[i]| < 1e-8),
+// computed once per ray: the reference's per-brick 1/dl and 1/|dl| are the
+// same correctly rounded values (RN(1/|x|) = |RN(1/x)|).
 template <bool COUNT>
 __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, const int bc[3], const float ol[3],
-                                          const float dl[3], float tstart, float texit, vx_hit& h,
-                                          TraceCounts& cnt)
+                                          const float dl[3], const float rinv[3], float tstart, float texit,
+                                          vx_hit& h, TraceCounts& cnt)
 {
     float bmin[3], tn[3], tf[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         bmin[i] = (float)(bc[i] * 8);
-        const float invd = (__builtin_fabsf(dl[i]) < 1e-8f) ? ((dl[i] >= 0.0f) ? 1e8f : -1e8f) : 1.0f / dl[i];
         const float bmax = bmin[i] + 8.0f;
-        const float t0 = (bmin[i] - ol[i]) * invd;
-        const float t1 = (bmax - ol[i]) * invd;
+        const float t0 = (bmin[i] - ol[i]) * rinv[i];
+        const float t1 = (bmax - ol[i]) * rinv[i];
         tn[i] = mn(t0, t1);
         tf[i] = mx(t0, t1);
     }
@@ -203,30 +213,21 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     if (btmin > btmax) return false;
 
     int cur[3], step[3];
-    float tDelta[3], tNext[3], entry[3];
+    float tDelta[3], tNext[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        entry[i] = ol[i] + dl[i] * btmin;
-        const int c = (int)__builtin_floorf(entry[i] - bmin[i]);
+        const float entry = ol[i] + dl[i] * btmin;
+        const int c = (int)__builtin_floorf(entry - bmin[i]);
         cur[i] = c < 0 ? 0 : (c > 7 ? 7 : c);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        if (__builtin_fabsf(dl[i]) < 1e-8f) {
-            step[i] = 0;
-            tDelta[i] = 3.402823466e+38f;
-            tNext[i] = 3.402823466e+38f;
-        } else if (dl[i] > 0.0f) {
-            step[i] = 1;
-            tDelta[i] = 1.0f / __builtin_fabsf(dl[i]);
-            const float nb = bmin[i] + (float)(cur[i] + 1);
-            tNext[i] = btmin + (nb - entry[i]) / dl[i];
-        } else {
-            step[i] = -1;
-            tDelta[i] = 1.0f / __builtin_fabsf(dl[i]);
-            const float nb = bmin[i] + (float)cur[i];
-            tNext[i] = btmin + (entry[i] - nb) / __builtin_fabsf(dl[i]);
-        }
+        // positive: (bmin + cur + 1 - entry) / dl; negative: (entry - (bmin + cur)) / |dl|;
+        // one division by |dl| either way, no divergent branch
+        const bool pos = dl[i] > 0.0f;
+        const bool flat = __builtin_fabsf(dl[i]) < 1e-8f;
+        const float nb = bmin[i] + (float)(cur[i] + (pos ? 1 : 0));
+        const float q = (pos ? nb - entry : entry - nb) / __builtin_fabsf(dl[i]);
+        step[i] = flat ? 0 : (pos ? 1 : -1);
+        tDelta[i] = flat ? 3.402823466e+38f : __builtin_fabsf(rinv[i]);
+        tNext[i] = flat ? 3.402823466e+38f : btmin + q;
     }
     // cur starts inside the brick (clamped above). Each pass steps the DDA first
     // (the reference steps after the test: the same voxel order, the same exits at
@@ -268,18 +269,16 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     }
     if (!hit) return false;
 
+    // Entry time of the hit voxel per axis: the reference takes min(t0, t1) of
+    // its two faces; rounding is monotone, so that is the near face's quotient
+    // ((vmin - o)/d for d > 0, (vmax - o)/d for d < 0): one division per axis.
     float tnh[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        if (__builtin_fabsf(dl[i]) < 1e-8f) {
-            tnh[i] = -__builtin_inff();
-        } else {
-            const float vmin = bmin[i] + (float)cur[i];
-            const float vmax = vmin + 1.0f;
-            const float t0 = (vmin - ol[i]) / dl[i];
-            const float t1 = (vmax - ol[i]) / dl[i];
-            tnh[i] = mn(t0, t1);
-        }
+        const float vmin = bmin[i] + (float)cur[i];
+        const float vmax = vmin + 1.0f;
+        const float tq = ((dl[i] > 0.0f ? vmin : vmax) - ol[i]) / dl[i];
+        tnh[i] = __builtin_fabsf(dl[i]) < 1e-8f ? -__builtin_inff() : tq;
     }
     float tHit = mx(mx(tnh[0], tnh[1]), tnh[2]);
     tHit = mx(tHit, tstart);
@@ -308,9 +307,10 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
 
 // Descriptor fetch: a prefix of the descriptor array (the top levels of a
 // breadth-first tree) may be staged in LDS by the launching kernel.
-// The launching kernel's dynamic LDS: staged descriptors first, then the
-// traversal stacks (VX_STACK parent pointers and VX_STACK t_max words per lane,
-// word (level * blockDim.x + threadIdx.x) of each half).
+// The launching kernel's dynamic LDS: staged descriptors first, then per lane
+// the traversal stack (stack_depth parent pointers, then stack_depth t_max
+// words) and the kAuxWords brick-DDA inputs; word j of a lane's column is at
+// base + j * blockDim.x + threadIdx.x.
 extern __shared__ uint2 vx_lds_nodes[];
 
 // STAGED is a compile-time choice: without staging the fetch is a plain global
@@ -322,10 +322,13 @@ __device__ __forceinline__ uint2 fetch_node(const SceneView& s, uint32_t i)
     return s.nodes[i];
 }
 
+__device__ __forceinline__ uint32_t* lane_word(uint32_t base, uint32_t j)
+{
+    return reinterpret_cast<uint32_t*>(vx_lds_nodes) + base + j * blockDim.x + threadIdx.x;
+}
 __device__ __forceinline__ uint32_t* stack_word(const SceneView& s, int k, bool tmax)
 {
-    uint32_t* words = reinterpret_cast<uint32_t*>(vx_lds_nodes);
-    return words + s.stack_base + ((tmax ? VX_STACK : 0) + k) * blockDim.x + threadIdx.x;
+    return lane_word(s.stack_base, (tmax ? s.stack_depth : 0u) + (uint32_t)k);
 }
 __device__ __forceinline__ void stack_put(const SceneView& s, int k, uint32_t ptr, float t)
 {
@@ -335,15 +338,24 @@ __device__ __forceinline__ void stack_put(const SceneView& s, int k, uint32_t pt
 __device__ __forceinline__ float stack_tmax(const SceneView& s, int k) { return u2f(*stack_word(s, k, true)); }
 __device__ __forceinline__ uint32_t stack_ptr(const SceneView& s, int k) { return *stack_word(s, k, false); }
 
-// LDS bytes a kernel needs for `lanes` traversal stacks.
-constexpr uint32_t stack_bytes(uint32_t lanes) { return lanes * VX_STACK * 2u * 4u; }
+// Per-lane LDS words beside the stack: the brick-DDA inputs ol[3], dl[3],
+// rinv[3], tstart, texit, then a pending leaf's brick id, brick cell (10 bits
+// per axis) and leaf descriptor index.
+constexpr uint32_t kAuxWords = 14;
+// LDS bytes per lane: stack (2 words per entry) and brick-DDA inputs.
+constexpr uint32_t lane_lds_bytes(uint32_t stack_depth) { return (2u * stack_depth + kAuxWords) * 4u; }
+// Lays out the per-lane columns after `staged` descriptors; returns the kernel's LDS bytes.
+inline uint32_t lay_out_lanes(SceneView& v, uint32_t staged, uint32_t lanes)
+{
+    v.stack_base = staged * 2u;
+    v.aux_base = v.stack_base + 2u * v.stack_depth * lanes;
+    return staged * 8u + lanes * lane_lds_bytes(v.stack_depth);
+}
 
 // Per-lane traversal state between iterations of the ESVO loop. Holding it in
 // a struct lets a kernel suspend a ray after any iteration (persistent kernels
 // refill retired lanes between iterations).
 struct RayState {
-    float ol[3], dl[3];      // local-space ray (voxel units)
-    float tstart, texit;     // traced segment
     float tc[3], tb[3];      // ESVO coefficients
     float pos[3];
     float scale_exp2, t_min, t_max;
@@ -358,12 +370,21 @@ __device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], c
                                          float tmax, RayState& r, vx_hit& h)
 {
     store_miss(h, 0u);
-    float du[3];
-    if (!setup_ray(s, o, d, tmin, tmax, r.ol, r.dl, du, r.tstart, r.texit)) return true;
+    float ol[3], dl[3], du[3], invd[3], tstart, texit;
+    if (!setup_ray(s, o, d, tmin, tmax, ol, dl, du, tstart, texit, invd)) return true;
+    // brick-DDA inputs to this lane's LDS column (read back at brick visits)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        *lane_word(s.aux_base, i) = f2u(ol[i]);
+        *lane_word(s.aux_base, 3 + i) = f2u(dl[i]);
+        *lane_word(s.aux_base, 6 + i) = f2u(__builtin_fabsf(dl[i]) < 1e-8f ? (dl[i] >= 0.0f ? 1e8f : -1e8f) : invd[i]);
+    }
+    *lane_word(s.aux_base, 9) = f2u(tstart);
+    *lane_word(s.aux_base, 10) = f2u(texit);
 
     float n0[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) n0[i] = (r.ol[i] + r.dl[i] * r.tstart) * s.invN + 1.0f;
+    for (int i = 0; i < 3; ++i) n0[i] = (ol[i] + dl[i] * tstart) * s.invN + 1.0f;
 
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -424,80 +445,30 @@ __device__ __forceinline__ float corrected_tc_max_f(float tx, float ty, float tz
     return mn(mn(vx, vy), vz);
 }
 
-// One iteration of the ESVO loop (VoxelRayMarch_Compressed.comp:396-478).
-// Returns true when the ray is finished; h then holds its record.
-template <bool COUNT, bool STAGED>
-__device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
+// One iteration of the ESVO loop (VoxelRayMarch_Compressed.comp:396-478) is
+// split at the brick visit so a wave can postpone it (trace_ray):
+//   esvo_step          the iteration up to the leaf test: PUSH, or ADVANCE/POP
+//                      for a skipped child, or STEP_LEAF when the child is a
+//                      brick (its brick id, cell and leaf index are returned;
+//                      the ray state is untouched);
+//   visit_leaf         the brick DDA of a leaf (hit: the ray is done);
+//   resume_after_leaf  the rest of that iteration after a brick miss (ADVANCE).
+// Chained without a pause (step_ray) they are exactly the reference iteration.
+enum : int { STEP_CONTINUE = 0, STEP_DONE = 1, STEP_LEAF = 2 };
+
+// A leaf child about to be visited: brick id, brick cell (10 signed bits per
+// axis) and the leaf descriptor's index.
+struct LeafRef {
+    uint32_t brick, cell, lidx;
+};
+
+// ADVANCE and POP of the iteration that started at r.iter, given the current
+// child's exit times tx, ty, tz and corrected max tcm. Returns true when the
+// ray is finished (a miss is written to h).
+__device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, vx_hit& h, float tx, float ty, float tz,
+                                            float tcm)
 {
     const int iter = r.iter;
-    if (!(iter < MAX_ITERS && r.scale <= ESVO_MAX_SCALE)) {
-        store_miss(h, (uint32_t)iter);
-        return true;
-    }
-    if (COUNT) cnt.iters += 1;
-    if (r.parent >= s.node_count) {
-        store_miss(h, (uint32_t)(iter + 1));
-        return true;
-    }
-    const uint2 desc = fetch_node<STAGED>(s, r.parent);
-    const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
-    const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
-    const int octm = r.mask & 7;
-    const int local = r.idx ^ ((~octm) & 7);
-
-    // The child's exit times serve both the PUSH test and ADVANCE (neither pos,
-    // mask nor t_max changes in between), so they are evaluated once, unconditionally.
-    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
-    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
-    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
-    const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
-    const float tv_max = mn(r.t_max, tcm);
-    const float half = r.scale_exp2 * 0.5f;
-    const float cx = half * r.tc[0] + tx;
-    const float cy = half * r.tc[1] + ty;
-    const float cz = half * r.tc[2] + tz;
-    const bool process = ((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS) && r.t_min <= tv_max + T_EPS;
-    if (process) {
-        if ((leaf >> local) & 1u) {
-            if (COUNT) cnt.leaves += 1;
-            const uint32_t below = (1u << local) - 1u;
-            const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
-            if (lidx < s.node_count) {
-                const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
-                if (brick != 0xFFFFFFu && brick < s.brick_count) {
-                    int bc[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
-                        bc[i] = (int)((lp - 1.0f) * s.Bf);
-                    }
-                    if (brick_dda<COUNT>(s, brick, bc, r.ol, r.dl, r.tstart, r.texit, h, cnt)) {
-                        h.flags |= (uint32_t)(iter + 1) << 16;
-                        h.leaf_node = lidx;
-                        if (COUNT) cnt.hits += 1;
-                        return true;
-                    }
-                }
-            }
-            r.t_min = tv_max;
-        } else {
-            stack_put(s, ESVO_MAX_SCALE - r.scale, r.parent, r.t_max);
-            const uint32_t below = (1u << local) - 1u;
-            r.parent = cp + popc8(valid & ~leaf & below);
-            int idx = 0;
-            r.scale -= 1;
-            r.scale_exp2 = half;
-            if (cx > r.t_min) { idx ^= 1; r.pos[0] += r.scale_exp2; }
-            if (cy > r.t_min) { idx ^= 2; r.pos[1] += r.scale_exp2; }
-            if (cz > r.t_min) { idx ^= 4; r.pos[2] += r.scale_exp2; }
-            r.idx = idx;
-            r.t_max = tv_max;
-            r.iter = iter + 1;
-            return false;
-        }
-    }
-
-    // ADVANCE
     const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
     const float tca =
         tcm >= 1e10f ? mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f) : tcm;
@@ -548,14 +519,164 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
     return false;
 }
 
-// Full traversal of one ray. Writes the hit record; returns true on a hit.
+// Brick DDA of a leaf child. True on a hit (h filled).
+template <bool COUNT>
+__device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r, uint32_t brick, const int bc[3],
+                                           uint32_t lidx, vx_hit& h, TraceCounts& cnt)
+{
+    float ol[3], dl[3], rinv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ol[i] = u2f(*lane_word(s.aux_base, i));
+        dl[i] = u2f(*lane_word(s.aux_base, 3 + i));
+        rinv[i] = u2f(*lane_word(s.aux_base, 6 + i));
+    }
+    const float tstart = u2f(*lane_word(s.aux_base, 9)), texit = u2f(*lane_word(s.aux_base, 10));
+    if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, h, cnt)) return false;
+    h.flags |= (uint32_t)(r.iter + 1) << 16;
+    h.leaf_node = lidx;
+    if (COUNT) cnt.hits += 1;
+    return true;
+}
+
+// IMMEDIATE: visit a leaf's brick inside the iteration (a miss falls through
+// to ADVANCE, as in the reference); otherwise return STEP_LEAF with lf set.
+template <bool COUNT, bool STAGED, bool IMMEDIATE>
+__device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt, LeafRef& lf)
+{
+    const int iter = r.iter;
+    if (!(iter < MAX_ITERS && r.scale <= ESVO_MAX_SCALE)) {
+        store_miss(h, (uint32_t)iter);
+        return STEP_DONE;
+    }
+    if (COUNT) cnt.iters += 1;
+    if (r.parent >= s.node_count) {
+        store_miss(h, (uint32_t)(iter + 1));
+        return STEP_DONE;
+    }
+    const uint2 desc = fetch_node<STAGED>(s, r.parent);
+    const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
+    const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
+    const int octm = r.mask & 7;
+    const int local = r.idx ^ ((~octm) & 7);
+
+    // The child's exit times serve both the PUSH test and ADVANCE (neither pos,
+    // mask nor t_max changes in between), so they are evaluated once, unconditionally.
+    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
+    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
+    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
+    const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
+    const float tv_max = mn(r.t_max, tcm);
+    const float half = r.scale_exp2 * 0.5f;
+    const float cx = half * r.tc[0] + tx;
+    const float cy = half * r.tc[1] + ty;
+    const float cz = half * r.tc[2] + tz;
+    const bool process = ((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS) && r.t_min <= tv_max + T_EPS;
+    if (process) {
+        if ((leaf >> local) & 1u) {
+            if (COUNT) cnt.leaves += 1;
+            const uint32_t below = (1u << local) - 1u;
+            const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
+            if (lidx < s.node_count) {
+                const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
+                if (brick != 0xFFFFFFu && brick < s.brick_count) {
+                    int bc[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
+                        bc[i] = (int)((lp - 1.0f) * s.Bf);
+                    }
+                    if (IMMEDIATE) {
+                        if (visit_leaf<COUNT>(s, r, brick, bc, lidx, h, cnt)) return STEP_DONE;
+                    } else {
+                        lf = LeafRef{brick, (bc[0] & 1023u) | ((bc[1] & 1023u) << 10) | ((bc[2] & 1023u) << 20), lidx};
+                        return STEP_LEAF;
+                    }
+                }
+            }
+            // no hit in the leaf (or no usable brick): the reference goes on to
+            // ADVANCE (its t_min = tv_max is overwritten there unread)
+        } else {
+            stack_put(s, ESVO_MAX_SCALE - r.scale, r.parent, r.t_max);
+            const uint32_t below = (1u << local) - 1u;
+            r.parent = cp + popc8(valid & ~leaf & below);
+            int idx = 0;
+            r.scale -= 1;
+            r.scale_exp2 = half;
+            if (cx > r.t_min) { idx ^= 1; r.pos[0] += r.scale_exp2; }
+            if (cy > r.t_min) { idx ^= 2; r.pos[1] += r.scale_exp2; }
+            if (cz > r.t_min) { idx ^= 4; r.pos[2] += r.scale_exp2; }
+            r.idx = idx;
+            r.t_max = tv_max;
+            r.iter = iter + 1;
+            return STEP_CONTINUE;
+        }
+    }
+    return advance_ray(s, r, h, tx, ty, tz, tcm) ? STEP_DONE : STEP_CONTINUE;
+}
+
+// The rest of the iteration after a brick miss: ADVANCE from the leaf child
+// (its exit times are recomputed from the unchanged state).
+__device__ __forceinline__ bool resume_after_leaf(const SceneView& s, RayState& r, vx_hit& h)
+{
+    const float tx = r.pos[0] * r.tc[0] - r.tb[0];
+    const float ty = r.pos[1] * r.tc[1] - r.tb[1];
+    const float tz = r.pos[2] * r.tc[2] - r.tb[2];
+    return advance_ray(s, r, h, tx, ty, tz, corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max));
+}
+
+// One whole reference iteration. Returns true when the ray is finished; h then
+// holds its record.
 template <bool COUNT, bool STAGED>
+__device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
+{
+    LeafRef lf;
+    return esvo_step<COUNT, STAGED, true>(s, r, h, cnt, lf) == STEP_DONE;
+}
+
+// Full traversal of one ray per lane. Writes the hit record; returns true on a
+// hit. Each lane executes exactly the reference's sequence of iterations.
+//
+// POSTPONE (for incoherent rays): a lane whose iteration reached a brick waits
+// until at least 3/4 of the wave's live lanes have, then those lanes run their
+// brick DDAs together, so the wave pays for the DDA loop once per batch of
+// leaves instead of once per lane's timing. Coherent camera rays reach leaves
+// together anyway; for them the immediate form (no bookkeeping) is faster.
+template <bool COUNT, bool STAGED, bool POSTPONE>
 __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
                                           float tmax, vx_hit& h, TraceCounts& cnt)
 {
     RayState r;
-    if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
-    while (!step_ray<COUNT, STAGED>(s, r, h, cnt)) {
+    if (!POSTPONE) {
+        if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
+        while (!step_ray<COUNT, STAGED>(s, r, h, cnt)) {
+        }
+        return (h.flags & 1u) != 0u;
+    }
+    enum : int { ESVO = 0, LEAF = 1, DONE = 2 };
+    int mode = init_ray(s, o, d, tmin, tmax, r, h) ? DONE : ESVO;
+    for (;;) {
+        if (mode == ESVO) {
+            LeafRef lf;
+            const int st = esvo_step<COUNT, STAGED, false>(s, r, h, cnt, lf);
+            mode = st == STEP_DONE ? DONE : (st == STEP_LEAF ? LEAF : ESVO);
+            if (mode == LEAF) { // parked in the lane's LDS column while it waits
+                *lane_word(s.aux_base, 11) = lf.brick;
+                *lane_word(s.aux_base, 12) = lf.cell;
+                *lane_word(s.aux_base, 13) = lf.lidx;
+            }
+        }
+        const uint64_t live = __ballot(mode != DONE);
+        if (live == 0ull) break;
+        const uint32_t waiting = (uint32_t)__popcll(__ballot(mode == LEAF));
+        // finished lanes, still in the loop, and lanes still traversing stay out
+        if (waiting * 4u >= (uint32_t)__popcll(live) * 3u && mode == LEAF) {
+            const uint32_t c = *lane_word(s.aux_base, 12);
+            const int bc[3] = {(int)(c << 22) >> 22, (int)(c << 12) >> 22, (int)(c << 2) >> 22}; // 10-bit signed
+            if (visit_leaf<COUNT>(s, r, *lane_word(s.aux_base, 11), bc, *lane_word(s.aux_base, 13), h, cnt))
+                mode = DONE;
+            else mode = resume_after_leaf(s, r, h) ? DONE : ESVO;
+        }
     }
     return (h.flags & 1u) != 0u;
 }
