@@ -14,15 +14,16 @@
 // Hits are either read back from the forward pass (8 bytes per pixel) or
 // recomputed by re-tracing (no per-pixel storage). Two reductions:
 //   atomic         one FP32 atomic add per hit pixel and channel;
-//   deterministic  (row, contribution) pairs are radix-sorted by row (stable,
-//                  so equal rows keep pixel order); the run heads are selected
-//                  (integer work) and one wave sums each run in a fixed order
-//                  (lane l takes elements l, l+64, ..., then a fixed butterfly),
-//                  so the per-row sums, added without conflicts, depend only on
-//                  the input: identical bits on every run. (A decoupled
-//                  look-back reduce-by-key is not used: the order in which it
-//                  combines the partial sums of a run spanning several tiles
-//                  depends on timing.)
+//   deterministic  (row, pixel) pairs are radix-sorted by row (stable, so
+//                  equal rows keep pixel order, misses last); the run heads
+//                  are selected (integer work) and each run is summed in a
+//                  fixed order (one thread in sequence for runs of <= 32 pairs,
+//                  else one workgroup: thread t takes elements t, t+256, ...,
+//                  then a fixed tree), so the per-row sums, added without
+//                  conflicts, depend only on the input: identical bits on
+//                  every run. (A decoupled look-back reduce-by-key is not
+//                  used: the order in which it combines the partial sums of a
+//                  run spanning several tiles depends on timing.)
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(kAlbBlock)
     k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
                      const float* __restrict__ grad_image, const uint32_t* __restrict__ hit_row,
                      const float* __restrict__ hit_shade, float* __restrict__ grad, uint32_t* __restrict__ keys,
-                     Rgb* __restrict__ vals)
+                     Rgb* __restrict__ vals, uint32_t* __restrict__ pix, uint32_t miss_key)
 {
     uint32_t px, py;
     const bool inside = tile_pixel(w, h, px, py);
@@ -202,8 +203,9 @@ __global__ void __launch_bounds__(kAlbBlock)
             atomicAdd(&grad[(size_t)row * 3 + 2], gb);
         }
     } else if (inside) {
-        keys[p] = row;
+        keys[p] = row == kNoVoxel ? miss_key : row; // misses sort last
         vals[p] = Rgb{gr, gg, gb};
+        pix[p] = (uint32_t)p;
     }
 }
 
@@ -214,43 +216,99 @@ struct RunHead {
     __host__ __device__ bool operator()(uint32_t i) const { return i == 0u || keys[i] != keys[i - 1u]; }
 };
 
-// One wave per run (grid-stride over the runs): lane l adds elements l, l+64,
-// ... of the run in order, a fixed butterfly combines the lanes and lane 0 adds
-// the sum into the run's row (rows are unique across runs: no conflicts). The
-// last run of misses (key ~0u) is skipped.
-__global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __restrict__ keys_sorted,
-                                                          const Rgb* __restrict__ vals_sorted,
-                                                          const uint32_t* __restrict__ heads,
-                                                          const uint32_t* __restrict__ n_runs, uint32_t n_items,
-                                                          float* __restrict__ grad)
+// Fixed-order run sums. Every run's sum is a function of its elements and
+// length only, so the gradient bits do not depend on scheduling. Runs of at
+// most kShortRun pairs: one thread adds them in order. Longer runs (voxels that
+// cover many pixels) are listed for k_reduce_long_runs. The last run, of
+// misses (key miss_key), is skipped. Rows are unique across runs: no conflicts.
+constexpr uint32_t kShortRun = 32;
+
+__device__ __forceinline__ uint32_t run_end(const uint32_t* heads, uint32_t r, uint32_t n, uint32_t n_items)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = (gridDim.x * kAlbBlock) >> 6;
-    const uint32_t n = *n_runs;
-    for (uint32_t r = (blockIdx.x * kAlbBlock + threadIdx.x) >> 6; r < n; r += waves) {
-        const uint32_t b = heads[r], e = r + 1u < n ? heads[r + 1u] : n_items;
+    return r + 1u < n ? heads[r + 1u] : n_items;
+}
+
+__global__ void __launch_bounds__(kAlbBlock) k_reduce_short_runs(const uint32_t* __restrict__ keys_sorted,
+                                                                const Rgb* __restrict__ vals,
+                                                                const uint32_t* __restrict__ pix_sorted,
+                                                                const uint32_t* __restrict__ heads,
+                                                                uint32_t* __restrict__ counts, uint32_t n_items,
+                                                                uint32_t miss_key, float* __restrict__ grad,
+                                                                uint32_t* __restrict__ long_runs)
+{
+    const uint32_t n = counts[0];
+    for (uint32_t r = blockIdx.x * kAlbBlock + threadIdx.x; r < n; r += gridDim.x * kAlbBlock) {
+        const uint32_t b = heads[r], e = run_end(heads, r, n, n_items);
         const uint32_t row = keys_sorted[b];
-        if (row == kNoVoxel) continue; // the same r on every lane: uniform
+        if (row >= miss_key) continue;
+        if (e - b > kShortRun) {
+            long_runs[atomicAdd(&counts[1], 1u)] = r; // list order is irrelevant to the sums
+            continue;
+        }
         float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-        for (uint32_t i = b + lane; i < e; i += 64u) {
-            const Rgb v = vals_sorted[i];
+        for (uint32_t i = b; i < e; ++i) {
+            const Rgb v = vals[pix_sorted[i]];
             sr += v.r;
             sg += v.g;
             sb += v.b;
         }
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            sr += __shfl_xor(sr, m, 64);
-            sg += __shfl_xor(sg, m, 64);
-            sb += __shfl_xor(sb, m, 64);
-        }
-        if (lane == 0u) {
-            grad[(size_t)row * 3 + 0] += sr;
-            grad[(size_t)row * 3 + 1] += sg;
-            grad[(size_t)row * 3 + 2] += sb;
-        }
+        grad[(size_t)row * 3 + 0] += sr;
+        grad[(size_t)row * 3 + 1] += sg;
+        grad[(size_t)row * 3 + 2] += sb;
     }
 }
+
+// One workgroup per listed long run: thread t adds elements t, t+256, t+512,
+// ... in order into two interleaved partial sums (independent loads in flight),
+// then the threads' sums are combined by a fixed tree in LDS.
+__global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* __restrict__ keys_sorted,
+                                                               const Rgb* __restrict__ vals,
+                                                               const uint32_t* __restrict__ pix_sorted,
+                                                               const uint32_t* __restrict__ heads,
+                                                               const uint32_t* __restrict__ counts, uint32_t n_items,
+                                                               const uint32_t* __restrict__ long_runs,
+                                                               float* __restrict__ grad)
+{
+    __shared__ float part[3][kAlbBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = counts[0], n_long = counts[1];
+    for (uint32_t j = blockIdx.x; j < n_long; j += gridDim.x) {
+        const uint32_t r = long_runs[j]; // the same j, so the same run, in the whole group
+        const uint32_t b = heads[r], e = run_end(heads, r, n, n_items);
+        float s0[3] = {0.0f, 0.0f, 0.0f}, s1[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t i = b + t;
+        for (; i + kAlbBlock < e; i += 2u * kAlbBlock) {
+            const Rgb v0 = vals[pix_sorted[i]], v1 = vals[pix_sorted[i + kAlbBlock]];
+            s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
+            s1[0] += v1.r; s1[1] += v1.g; s1[2] += v1.b;
+        }
+        if (i < e) {
+            const Rgb v0 = vals[pix_sorted[i]];
+            s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
+        }
+        part[0][t] = s0[0] + s1[0];
+        part[1][t] = s0[1] + s1[1];
+        part[2][t] = s0[2] + s1[2];
+        __syncthreads();
+        for (uint32_t h = kAlbBlock / 2u; h > 0u; h >>= 1) {
+            if (t < h) {
+                part[0][t] += part[0][t + h];
+                part[1][t] += part[1][t + h];
+                part[2][t] += part[2][t + h];
+            }
+            __syncthreads();
+        }
+        if (t == 0u) {
+            const uint32_t row = keys_sorted[b];
+            grad[(size_t)row * 3 + 0] += part[0][0];
+            grad[(size_t)row * 3 + 1] += part[1][0];
+            grad[(size_t)row * 3 + 2] += part[2][0];
+        }
+        __syncthreads(); // part[] is reused by the next run
+    }
+}
+
+__global__ void k_clear_long_count(uint32_t* counts) { counts[1] = 0u; }
 
 } // namespace vx
 
@@ -260,8 +318,10 @@ struct vx_albedo_state {
     uint64_t voxels = 0;
     // sorted-reduction workspace, grown on demand
     size_t cap_px = 0, cap_tmp = 0;
-    uint32_t *keys = nullptr, *keys_sorted = nullptr, *heads = nullptr, *n_runs = nullptr;
-    vx::Rgb *vals = nullptr, *vals_sorted = nullptr;
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *heads = nullptr;
+    uint32_t* n_runs = nullptr; // [0] runs, [1] long runs
+    vx::Rgb* vals = nullptr;                         // contributions, by pixel
+    uint32_t *pix = nullptr, *pix_sorted = nullptr; // pixel index, sorted with the keys
     void* tmp = nullptr;
 };
 
@@ -336,13 +396,13 @@ static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
 {
     if (px > a->cap_px) {
         for (void* p : {(void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs, (void*)a->vals,
-                        (void*)a->vals_sorted})
+                        (void*)a->pix, (void*)a->pix_sorted})
             (void)hipFree(p);
         a->cap_px = 0;
         if (hipMalloc(&a->keys, px * 4) != hipSuccess || hipMalloc(&a->keys_sorted, px * 4) != hipSuccess ||
-            hipMalloc(&a->heads, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 4) != hipSuccess ||
-            hipMalloc(&a->vals, px * sizeof(Rgb)) != hipSuccess ||
-            hipMalloc(&a->vals_sorted, px * sizeof(Rgb)) != hipSuccess)
+            hipMalloc(&a->heads, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 8) != hipSuccess ||
+            hipMalloc(&a->vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&a->pix, px * 4) != hipSuccess ||
+            hipMalloc(&a->pix_sorted, px * 4) != hipSuccess)
             return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
         a->cap_px = px;
     }
@@ -361,7 +421,7 @@ void vx_albedo_state_free(vx_albedo_state* a)
 {
     if (!a) return;
     for (void* p : {(void*)a->base, (void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs,
-                    (void*)a->vals, (void*)a->vals_sorted, a->tmp})
+                    (void*)a->vals, (void*)a->pix, (void*)a->pix_sorted, a->tmp})
         (void)hipFree(p);
     delete a;
 }
@@ -421,44 +481,54 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
     if (mode == VX_GRAD_ATOMIC) {
         if (recompute)
             vx::k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr,
-                                                                       nullptr, grad_albedo, nullptr, nullptr);
+                                                                       nullptr, grad_albedo, nullptr, nullptr, nullptr, 0u);
         else
             vx::k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
-                                                                      hit_shade, grad_albedo, nullptr, nullptr);
+                                                                      hit_shade, grad_albedo, nullptr, nullptr, nullptr,
+                                                                      0u);
         if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
         return VX_OK;
     }
     // deterministic: sort (row, contribution) by row, select the run heads, sum each run in a fixed order
     if (px > 0x7FFFFFFFu) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: deterministic mode needs w*h < 2^31");
-    const int end_bit = 32; // misses carry the all-ones key and sort last
+    if (a->voxels >= 0xFFFFFFFFull) return vx::fail(VX_ERR_UNSUPPORTED, "vx_albedo_backward: too many voxels");
+    // misses get key V (the voxel count) and sort last; the sort looks at the key bits V needs
+    const uint32_t miss_key = (uint32_t)a->voxels;
+    int end_bit = 1;
+    while (end_bit < 32 && (miss_key >> end_bit) != 0u) ++end_bit;
     size_t sort_bytes = 0, red_bytes = 0;
     const hipcub::CountingInputIterator<uint32_t> index(0u);
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (vx::Rgb*)nullptr, (vx::Rgb*)nullptr, (int)px, 0, end_bit, st) != hipSuccess ||
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px, 0, end_bit, st) != hipSuccess ||
         hipcub::DeviceSelect::If(nullptr, red_bytes, index, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px,
                                  vx::RunHead{nullptr}, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: workspace query failed");
     if (vx_status g = vx::grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
     if (recompute)
         vx::k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
-                                                                    nullptr, a->keys, a->vals);
+                                                                    nullptr, a->keys, a->vals, a->pix, miss_key);
     else
         vx::k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
-                                                                   hit_shade, nullptr, a->keys, a->vals);
+                                                                   hit_shade, nullptr, a->keys, a->vals, a->pix,
+                                                                   miss_key);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
     size_t tb = a->cap_tmp;
-    if (hipcub::DeviceRadixSort::SortPairs(a->tmp, tb, a->keys, a->keys_sorted, a->vals, a->vals_sorted, (int)px, 0,
+    if (hipcub::DeviceRadixSort::SortPairs(a->tmp, tb, a->keys, a->keys_sorted, a->pix, a->pix_sorted, (int)px, 0,
                                            end_bit, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: sort failed");
     tb = a->cap_tmp;
     if (hipcub::DeviceSelect::If(a->tmp, tb, index, a->heads, a->n_runs, (int)px, vx::RunHead{a->keys_sorted}, st) !=
         hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run selection failed");
-    // a few resident groups per CU walk all runs (their count stays on the device)
+    // resident groups walk all runs (their count stays on the device); the keys
+    // buffer, consumed by the sort, holds the list of long runs
     const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->ctx->cu_count * 8u,
-                                                                               (px + 3) / 4));
-    vx::k_reduce_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals_sorted, a->heads, a->n_runs,
-                                                      (uint32_t)px, grad_albedo);
+                                                                               (px + vx::kAlbBlock - 1) / vx::kAlbBlock));
+    vx::k_clear_long_count<<<1, 1, 0, st>>>(a->n_runs);
+    vx::k_reduce_short_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals, a->pix_sorted, a->heads, a->n_runs,
+                                                            (uint32_t)px, miss_key, grad_albedo, a->keys);
+    vx::k_reduce_long_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals, a->pix_sorted, a->heads, a->n_runs,
+                                                           (uint32_t)px, a->keys, grad_albedo);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run reduction failed");
     return VX_OK;
 }
