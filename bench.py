@@ -16,6 +16,9 @@ Two more legs ride in the same JSON line:
   fwd_bwd   differentiable albedo on the 256^3 cave, 4 seeded 1280x720 views:
             forward (render + per-voxel albedo shading) and the adjoint, timed
             per view, and their ratio.
+  incoherent  vx_trace on the same scene with incoherent rays: 2M seeded
+            random rays from inside the volume, and one seeded cosine bounce
+            ray per primary hit of the headline frame.
 Also:
   roofline      algorithmic bytes per launch / average launch time against the
                 8 TB/s HBM3E peak; per ray: 8*iterations + 8*brick visits +
@@ -229,6 +232,31 @@ def fwd_bwd_leg(vx, torch, ctx, stream, grid, built, steps: int):
             "fwd_bwd_over_plain_forward": round((ms_fwd + ms_bwd_det) / ms_plain, 3)}
 
 
+# ---------------------------------------------------------------------------
+# incoherent-ray leg
+# ---------------------------------------------------------------------------
+def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int):
+    g = torch.Generator(device="cuda").manual_seed(1)
+    n = 2_000_000
+    rr = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    rr[:, 0:3] = torch.rand((n, 3), generator=g, device="cuda") * N
+    rr[:, 3] = 0.0
+    rr[:, 4:7] = torch.randn((n, 3), generator=g, device="cuda")
+    rr[:, 7] = 3.0e38
+    rr = rr.view(torch.uint8).view(-1)
+    h_rand = torch.empty_like(rr)
+    primary = vx.camera_rays(cam, W, H, 1e30)
+    bounce = scene.bounce_rays(primary, scene.trace(primary), seed=7)
+    h_bounce = torch.empty_like(bounce)
+    ms_rand = timed(torch, stream, lambda: scene.trace(rr, h_rand, stream=stream), steps, 2)
+    ms_bounce = timed(torch, stream, lambda: scene.trace(bounce, h_bounce, stream=stream), steps, 2)
+    return {"workload": "cave256: 2M random rays from inside the volume (seed 1); one cosine bounce ray per "
+                        "primary hit of the 1080p frame (seed 7)",
+            "random_ms": round(ms_rand, 4), "random_mrays": round(n / ms_rand / 1e3, 1),
+            "bounce_ms": round(ms_bounce, 4), "bounce_rays": W * H,
+            "bounce_mrays": round(W * H / ms_bounce / 1e3, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +265,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
+    ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent-ray leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -304,6 +333,10 @@ def main() -> int:
         "roofline": roofline,
     }
     got = vx.hits_to_numpy(hits)
+    if not args.no_incoherent:
+        log("[bench] incoherent-ray leg...")
+        result["incoherent"] = incoherent_leg(vx, torch, scene, cam, W, H, N, stream, max(5, args.steps // 5))
+        log(f"[bench] incoherent: {result['incoherent']}")
     if not args.no_fwdbwd:
         log("[bench] fwd+bwd leg...")
         result["fwd_bwd"] = fwd_bwd_leg(vx, torch, ctx, stream, grid, built, max(3, args.steps // 5))

@@ -26,6 +26,7 @@
 //                  run spanning several tiles depends on timing.)
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,12 @@
 namespace vx {
 
 constexpr uint32_t kAlbBlock = 256;
+
+// LSD radix sort (onesweep) at every size: rocprim's default switches to a
+// block-sort + merge-pass algorithm below 2^20 items, which at one 1280x720
+// view costs about 20 launches instead of 3 onesweep passes over the key bits.
+using PairSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                  rocprim::default_config, 0>;
 constexpr uint32_t kNoVoxel = 0xFFFFFFFFu;
 
 struct Rgb {
@@ -498,8 +505,9 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
     while (end_bit < 32 && (miss_key >> end_bit) != 0u) ++end_bit;
     size_t sort_bytes = 0, red_bytes = 0;
     const hipcub::CountingInputIterator<uint32_t> index(0u);
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px, 0, end_bit, st) != hipSuccess ||
+    if (rocprim::radix_sort_pairs<vx::PairSortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                      (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)px, 0u,
+                                                      (unsigned)end_bit, st) != hipSuccess ||
         hipcub::DeviceSelect::If(nullptr, red_bytes, index, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px,
                                  vx::RunHead{nullptr}, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: workspace query failed");
@@ -513,8 +521,8 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
                                                                    miss_key);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
     size_t tb = a->cap_tmp;
-    if (hipcub::DeviceRadixSort::SortPairs(a->tmp, tb, a->keys, a->keys_sorted, a->pix, a->pix_sorted, (int)px, 0,
-                                           end_bit, st) != hipSuccess)
+    if (rocprim::radix_sort_pairs<vx::PairSortConfig>(a->tmp, tb, a->keys, a->keys_sorted, a->pix, a->pix_sorted,
+                                                      (uint32_t)px, 0u, (unsigned)end_bit, st) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: sort failed");
     tb = a->cap_tmp;
     if (hipcub::DeviceSelect::If(a->tmp, tb, index, a->heads, a->n_runs, (int)px, vx::RunHead{a->keys_sorted}, st) !=
