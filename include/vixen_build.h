@@ -52,6 +52,24 @@ void vx_built_scene_free(vx_built_scene* s);
  * validCount 16). */
 uint64_t vx_bc1_encode_block(const float colors[16][3]);
 
+/* The normal pool of VIXEN's compressed shader (compressedNormals, binding 7):
+ * per brick 32 blocks of 16 consecutive linear voxels, each block two u64
+ * {blockA = base normal | U selectors << 32, blockB = UV axes | V selectors << 32}
+ * at normals[brick*64 + 2*block]. Per voxel the geometric normal of the brick's
+ * occupancy (LaineKarrasOctree::rebuild, libraries/SVO/src/SVORebuild.cpp:130-220,
+ * 613-731), encoded by DXTNormalCompressor::encodeBlockTyped
+ * (libraries/VoxelData/src/Compression/DXT1Compressor.cpp:244-355); empty voxels
+ * contribute (0,1,0), all-empty blocks are {0,0}. brick_data is brick_count x 512
+ * words (!= 0 is occupied). nthreads <= 0 uses all cores. */
+vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, int nthreads, uint64_t* normals);
+
+/* DXTNormalCompressor::encodeBlockTyped of 16 normals (validCount 16). */
+void vx_dxt_normal_encode_block(const float normals[16][3], uint64_t out[2]);
+
+/* computeGeometricNormal (SVORebuild.cpp:173-193) of voxel (x,y,z) in one brick
+ * of 512 words: -normalize(occupancy gradient), or (0,1,0) when it vanishes. */
+void vx_geometric_normal(const uint32_t brick[512], int x, int y, int z, float out[3]);
+
 /* Synthetic scenes (voxel units, world = [0,N]^3). Each fills grid (N^3 words)
  * and a camera (vx_camera, fov in degrees) for the scene's primary view.
  *   cornell: N^3 room, 1-voxel walls x=0 red, x=N-1 green, y=0, y=N-1 and

@@ -82,6 +82,9 @@ typedef struct vx_ray {
  *   flags bit 0      : hit
  *   flags bits 1-2   : entry-face axis (0 x, 1 y, 2 z), valid on a hit
  *   flags bit 3      : entry-face normal sign is positive
+ *   flags bits 4-5   : axis of the brick-DDA step into the hit voxel (1 x, 2 y,
+ *                      3 z); 0 when the hit voxel is the first one the DDA tests
+ *                      in its brick (the compressed shader's axisMask)
  *   flags bits 16-31 : ESVO loop iterations executed (hit or miss)
  *   t                : world-space distance to the entry of the hit voxel's AABB,
  *                      clamped to the start of the traced segment; +inf on a miss
@@ -127,6 +130,8 @@ typedef struct vx_scene_desc {
     const uint64_t* bc1_colors;   /* brick_count x 32 BC1 blocks, or NULL: colour is
                                      then brick_data's RGBA8 (R in bits 0-7) */
     const vx_octree_config* config;
+    const uint64_t* dxt_normals;  /* brick_count x 32 DXT normal blocks of two u64
+                                     (compressedNormals, see vx_encode_normals), or NULL */
 } vx_scene_desc;
 
 typedef struct vx_ctx vx_ctx;
@@ -179,8 +184,16 @@ vx_status vx_trace(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits
  * one pinhole ray per pixel (shaders/RayGeneration.glsl:61-71, pixel centre
  * (p + 0.5) / size, Vulkan y-flip), traced over [0, tmax]. Writes a w*h hit
  * record image (row-major, y down) when hits != NULL and/or an RGBA8 image
- * shaded with computeLighting (shaders/Lighting.glsl:19-28) and the sky
- * gradient of VoxelRayMarch_Compressed.comp:541-548 when rgba8 != NULL. */
+ * (R in bits 0-7, A = 255) when rgba8 != NULL. Misses get the sky gradient of
+ * VoxelRayMarch_Compressed.comp:541-548. Hits are shaded with computeLighting
+ * (shaders/Lighting.glsl:19-28):
+ *   - when the scene has BC1 colours and DXT normals, as the compressed shader
+ *     does: colour from decodeDXT1Color (shaders/Compression.glsl:60-75),
+ *     normal = normalize(normalize(decodeDXTNormal) + 0.5 * DDA step normal),
+ *     or the ray's dominant-axis normal when the hit voxel is the first the
+ *     DDA tests in its brick (VoxelRayMarch_Compressed.comp:160-165,239-256);
+ *   - otherwise with the hit record's colour and entry-face normal.
+ * Float -> 8 bit: clamp to [0,1], then (uint)(x * 255 + 0.5). */
 vx_status vx_render(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
                     float tmax, vx_hit* hits, uint32_t* rgba8, vx_stream stream);
 

@@ -24,7 +24,8 @@ HIT_DTYPE = np.dtype([("flags", "<u4"), ("t", "<f4"), ("leaf_node", "<u4"), ("br
 class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("brick_data", C.c_void_p),
                 ("brick_count", C.c_uint64), ("bc1", C.c_void_p), ("grid_min", C.c_float * 3),
-                ("grid_max", C.c_float * 3), ("max_levels", C.c_int32), ("bricks_per_axis", C.c_int32)]
+                ("grid_max", C.c_float * 3), ("max_levels", C.c_int32), ("bricks_per_axis", C.c_int32),
+                ("normals", C.c_void_p)]
 
 
 def build() -> Path:
@@ -45,8 +46,13 @@ def _load() -> C.CDLL:
     lib.orc_bruteforce.argtypes = [C.POINTER(OrcScene), vp, vp, u64, C.c_double, vp, vp, vp]
     lib.orc_setup_ray.argtypes = [C.POINTER(OrcScene), vp, vp, vp, vp, vp, vp]
     lib.orc_setup_ray.restype = C.c_int
+    lib.orc_bc1_decode_shader.argtypes = [C.c_uint64, C.c_int, vp]
+    lib.orc_dxt_normal_decode.argtypes = [C.c_uint64, C.c_uint64, C.c_int, vp]
+    lib.orc_shade.argtypes = [C.POINTER(OrcScene), vp, vp, C.c_float]
+    lib.orc_shade.restype = C.c_uint32
+    lib.orc_render.argtypes = [C.POINTER(OrcScene), vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_int]
     for f in ("orc_trace", "orc_trace_mt", "orc_camera_rays", "orc_bc1_decode", "orc_rgba8_decode",
-              "orc_bruteforce"):
+              "orc_bruteforce", "orc_bc1_decode_shader", "orc_dxt_normal_decode", "orc_render"):
         getattr(lib, f).restype = None
     return lib
 
@@ -64,10 +70,11 @@ def lib() -> C.CDLL:
 class OracleScene:
     """Holds the arrays an orc_scene points at (nodes, bricks, BC1, bounds)."""
 
-    def __init__(self, nodes, brick_data, bc1, grid_min, grid_max, max_levels, bricks_per_axis):
+    def __init__(self, nodes, brick_data, bc1, grid_min, grid_max, max_levels, bricks_per_axis, normals=None):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         self.brick_data = np.ascontiguousarray(brick_data, dtype=np.uint32)
         self.bc1 = None if bc1 is None else np.ascontiguousarray(bc1, dtype=np.uint64)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.uint64)
         s = OrcScene()
         s.nodes = self.nodes.ctypes.data
         s.node_count = self.nodes.size // 2
@@ -78,13 +85,14 @@ class OracleScene:
         s.grid_max[:] = [float(x) for x in grid_max]
         s.max_levels = int(max_levels)
         s.bricks_per_axis = int(bricks_per_axis)
+        s.normals = self.normals.ctypes.data if (self.normals is not None and self.normals.size) else None
         self.s = s
 
     @classmethod
     def from_built(cls, built) -> "OracleScene":
         cfg = built.config
         return cls(built.nodes, built.brick_data, built.bc1, list(cfg.grid_min), list(cfg.grid_max),
-                   cfg.user_max_levels, cfg.bricks_per_axis)
+                   cfg.user_max_levels, cfg.bricks_per_axis, getattr(built, "normals", None))
 
 
 def trace(scene: OracleScene, rays: np.ndarray, nthreads: int = 1):
@@ -111,6 +119,26 @@ def bc1_decode(block: int, texel: int) -> np.ndarray:
     rgb = np.zeros(3, np.float32)
     lib().orc_bc1_decode(int(block), int(texel), rgb.ctypes.data)
     return rgb
+
+
+def bc1_decode_shader(block: int, texel: int) -> np.ndarray:
+    rgb = np.zeros(3, np.float32)
+    lib().orc_bc1_decode_shader(int(block), int(texel), rgb.ctypes.data)
+    return rgb
+
+
+def dxt_normal_decode(block_a: int, block_b: int, texel: int) -> np.ndarray:
+    n = np.zeros(3, np.float32)
+    lib().orc_dxt_normal_decode(int(block_a), int(block_b), int(texel), n.ctypes.data)
+    return n
+
+
+def render(scene: OracleScene, cam_floats: np.ndarray, w: int, h: int, tmax: float, nthreads: int = 8) -> np.ndarray:
+    """vx_render's RGBA8 image (uint32 per pixel, R in bits 0-7), row-major."""
+    cam = np.ascontiguousarray(cam_floats, dtype=np.float32)
+    out = np.zeros(w * h, np.uint32)
+    lib().orc_render(C.byref(scene.s), cam.ctypes.data, w, h, float(tmax), out.ctypes.data, nthreads)
+    return out
 
 
 def rgba8_decode(word: int) -> np.ndarray:

@@ -16,6 +16,12 @@
  *   orc_bc1_decode    pinned exhaustively: all 4 selectors x random heads
  *                     against the integer RGB565 closed form
  *   orc_camera_rays   pinned by closed-form pixel-centre directions
+ *   orc_bc1_decode_shader  pinned against orc_bc1_decode on random blocks x
+ *                     texels (the two formulas differ only below 1/32 of a step)
+ *   orc_dxt_normal_decode  pinned by the raw-normal bit layout and by
+ *                     encode -> decode round trips (tests/test_shading.py)
+ *   orc_shade         pinned by a float64 evaluation of every pixel of a frame
+ *                     (within one 8-bit step)
  *   orc_bruteforce    textbook Amanatides-Woo in FP64; pinned by random-ray
  *                     agreement with an exhaustive slab test per voxel on tiny grids
  */
@@ -192,6 +198,9 @@ static int brick_dda(const orc_scene* s, uint32_t brick, const int bc[3], const 
             tNext[i] = btmin + (entry[i] - nb) / fabsf(dl[i]);
         }
     }
+    /* axis of the step into the current voxel, 0 for the brick's entry voxel: the
+     * compressed shader's axisMask (VoxelRayMarch_Compressed.comp:213-275) */
+    uint32_t last = 0;
     for (int it = 0; it < 3 * BRICK_N; ++it) {
         if (cur[0] < 0 || cur[0] >= BRICK_N || cur[1] < 0 || cur[1] >= BRICK_N || cur[2] < 0 ||
             cur[2] >= BRICK_N)
@@ -219,7 +228,7 @@ static int brick_dda(const orc_scene* s, uint32_t brick, const int bc[3], const 
             else if (tnh[1] >= tnh[2]) axis = 1;
             else axis = 2;
             const uint32_t signpos = (dl[axis] > 0.0f) ? 0u : 1u;
-            h->flags = 1u | (axis << 1) | (signpos << 3);
+            h->flags = 1u | (axis << 1) | (signpos << 3) | (last << 4);
             h->t = tHit;
             h->brick = brick;
             h->voxel = lin;
@@ -231,14 +240,17 @@ static int brick_dda(const orc_scene* s, uint32_t brick, const int bc[3], const 
             if (tNext[0] > btmax) return 0;
             cur[0] += step[0];
             tNext[0] += tDelta[0];
+            last = 1;
         } else if (tNext[1] < tNext[2]) {
             if (tNext[1] > btmax) return 0;
             cur[1] += step[1];
             tNext[1] += tDelta[1];
+            last = 2;
         } else {
             if (tNext[2] > btmax) return 0;
             cur[2] += step[2];
             tNext[2] += tDelta[2];
+            last = 3;
         }
     }
     return 0;
@@ -477,6 +489,146 @@ void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* 
         if (counts)
             for (int k = 0; k < 5; ++k) counts[k] += jobs[t].c[k];
     }
+}
+
+/* ---------------------------------------------------------------------- */
+/* The compressed shader's per-pixel shading                               */
+/* ---------------------------------------------------------------------- */
+
+/* decodeDXT1Color (Compression.glsl:28-40, 60-75) */
+void orc_bc1_decode_shader(uint64_t block, int texel, float rgb[3])
+{
+    const float DXT_SCALE = 1.0f / 4294967296.0f;
+    const float coefs[4] = {DXT_SCALE, 0.0f, 2.0f * DXT_SCALE / 3.0f, 1.0f * DXT_SCALE / 3.0f};
+    const uint32_t head = (uint32_t)block, bits = (uint32_t)(block >> 32);
+    const float c0 = coefs[(bits >> (texel * 2)) & 3u];
+    const float c1 = DXT_SCALE - c0;
+    rgb[0] = c0 * (float)(head << 16) + c1 * (float)head;
+    rgb[1] = c0 * (float)(head << 21) + c1 * (float)(head << 5);
+    rgb[2] = c0 * (float)(head << 27) + c1 * (float)(head << 11);
+}
+
+/* decodeRawNormal (Compression.glsl:115-125) */
+static void decode_raw_normal(uint32_t value, float n[3])
+{
+    const int32_t sign = (int32_t)value >> 31;
+    const int axis = (int)((value >> 29) & 3u);
+    const int32_t t = sign ^ 0x7FFFFFFF;
+    const int32_t u = (int32_t)(value << 3);
+    const int32_t v = (int32_t)(value << 18);
+    if (axis == 0) { n[0] = (float)t; n[1] = (float)u; n[2] = (float)v; return; }
+    if (axis == 1) { n[0] = (float)v; n[1] = (float)t; n[2] = (float)u; return; }
+    n[0] = (float)u; n[1] = (float)v; n[2] = (float)t;
+}
+
+/* decodeNormalAxis (Compression.glsl:134-144) */
+static void decode_normal_axis(uint32_t head_uv, int shift, float a[3])
+{
+    const int32_t shifted = (int32_t)(head_uv << (16 - shift));
+    const int exponent = (shifted >> 16) & 15;
+    const float scale = exp2f((float)(exponent + (3 - 13)));
+    a[0] = (float)shifted * scale;
+    a[1] = (float)(int32_t)((uint32_t)shifted << 4) * scale;
+    a[2] = (float)(int32_t)((uint32_t)shifted << 8) * scale;
+}
+
+/* decodeDXTNormal (Compression.glsl:41, 159-177) */
+void orc_dxt_normal_decode(uint64_t block_a, uint64_t block_b, int texel, float n[3])
+{
+    const float coefs[4] = {-1.0f, -1.0f / 3.0f, 1.0f / 3.0f, 1.0f};
+    float base[3], u[3], v[3];
+    decode_raw_normal((uint32_t)block_a, base);
+    decode_normal_axis((uint32_t)block_b, 0, u);
+    decode_normal_axis((uint32_t)block_b, 16, v);
+    const float cu = coefs[((uint32_t)(block_a >> 32) >> (texel * 2)) & 3u];
+    const float cv = coefs[((uint32_t)(block_b >> 32) >> (texel * 2)) & 3u];
+    for (int i = 0; i < 3; ++i) n[i] = base[i] + u[i] * cu + v[i] * cv;
+}
+
+/* GLSL normalize as v * (1 / sqrt(dot(v, v))) */
+static void normalize3(float v[3])
+{
+    const float inv = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int i = 0; i < 3; ++i) v[i] = v[i] * inv;
+}
+
+static uint32_t to_u8(float x)
+{
+    x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    return (uint32_t)(x * 255.0f + 0.5f);
+}
+
+static float glsl_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+uint32_t orc_shade(const orc_scene* s, const orc_ray* r, const orc_hit* h, float v)
+{
+    float col[3];
+    if (!(h->flags & 1u)) {
+        /* sky: vec3(0.5, 0.7, 1.0) * (1.0 - uv.y * 0.5) (VoxelRayMarch_Compressed.comp:544) */
+        const float g = 1.0f - v * 0.5f;
+        col[0] = 0.5f * g;
+        col[1] = 0.7f * g;
+        col[2] = 1.0f * g;
+    } else {
+        /* computeLighting: lightDir = normalize(vec3(1, 1, -1)), folded to the
+         * correctly rounded 1/sqrt(3); color * (0.3 + max(dot(n, l), 0) * 0.7) */
+        const float L[3] = {0.57735026918962576f, 0.57735026918962576f, -0.57735026918962576f};
+        float rgb[3], n[3] = {0.0f, 0.0f, 0.0f};
+        if (s->normals && s->bc1) {
+            float ol[3], dl[3], du[3], ts, te;
+            orc_setup_ray(s, r, ol, dl, du, &ts, &te); /* the normalised world direction */
+            const uint64_t blk = (uint64_t)h->brick * 32u + (h->voxel >> 4);
+            orc_bc1_decode_shader(s->bc1[blk], (int)(h->voxel & 15u), rgb);
+            const uint32_t axis_mask = (h->flags >> 4) & 3u; /* 0: no DDA step in the hit brick */
+            if (axis_mask == 0u) {
+                /* VoxelRayMarch_Compressed.comp:241-249 */
+                const float ax = fabsf(du[0]), ay = fabsf(du[1]), az = fabsf(du[2]);
+                if (ax > ay && ax > az) n[0] = -glsl_sign(du[0]);
+                else if (ay > az) n[1] = -glsl_sign(du[1]);
+                else n[2] = -glsl_sign(du[2]);
+            } else {
+                /* :160-165 and :250-255: normalize(normalize(decoded) + ddaNormal * 0.5) */
+                orc_dxt_normal_decode(s->normals[2 * blk], s->normals[2 * blk + 1], (int)(h->voxel & 15u), n);
+                normalize3(n);
+                float dda[3] = {0.0f, 0.0f, 0.0f};
+                const int a = (int)axis_mask - 1;
+                const float st = du[a] < 0.0f ? -1.0f : 1.0f; /* ivec3(sign(rayDir)), 0 -> 1 */
+                dda[a] = -st;
+                for (int i = 0; i < 3; ++i) n[i] = n[i] + dda[i] * 0.5f;
+                normalize3(n);
+            }
+        } else {
+            const uint32_t axis = (h->flags >> 1) & 3u;
+            n[axis] = (h->flags & 8u) ? 1.0f : -1.0f; /* entry-face normal */
+            for (int i = 0; i < 3; ++i) rgb[i] = h->rgb[i];
+        }
+        const float d = n[0] * L[0] + n[1] * L[1] + n[2] * L[2];
+        const float k = 0.3f + (d < 0.0f ? 0.0f : d) * 0.7f;
+        for (int i = 0; i < 3; ++i) col[i] = rgb[i] * k;
+    }
+    return to_u8(col[0]) | (to_u8(col[1]) << 8) | (to_u8(col[2]) << 16) | 0xFF000000u;
+}
+
+void orc_render(const orc_scene* s, const float cam[16], uint32_t w, uint32_t h, float tmax, uint32_t* rgba,
+                int nthreads)
+{
+    const uint64_t n = (uint64_t)w * h;
+    orc_ray* rays = (orc_ray*)malloc(n * sizeof(orc_ray));
+    orc_hit* hits = (orc_hit*)malloc(n * sizeof(orc_hit));
+    if (!rays || !hits) {
+        free(rays);
+        free(hits);
+        return;
+    }
+    orc_camera_rays(cam, w, h, tmax, rays);
+    orc_trace_mt(s, rays, n, hits, NULL, nthreads);
+    for (uint32_t py = 0; py < h; ++py)
+        for (uint32_t px = 0; px < w; ++px) {
+            const uint64_t i = (uint64_t)py * w + px;
+            rgba[i] = orc_shade(s, &rays[i], &hits[i], ((float)py + 0.5f) / (float)h);
+        }
+    free(rays);
+    free(hits);
 }
 
 /* Pinhole rays: getRayDir (RayGeneration.glsl:61-71) at pixel centres

@@ -13,6 +13,9 @@
  *   - brick DDA, hit t and normal    libraries/SVO/src/SVOBrickDDA.cpp:187-225,390-533
  *   - BC1 colour decode              libraries/VoxelData/src/Compression/DXT1Compressor.cpp:14-36
  *   - pinhole ray generation         shaders/RayGeneration.glsl:61-71, VoxelRayMarch_Compressed.comp:501-503
+ *   - the compressed shader's shading shaders/Compression.glsl:28-41,60-75,115-177,
+ *                                    VoxelRayMarch_Compressed.comp:154-165,239-256,541-548,
+ *                                    shaders/Lighting.glsl:19-28
  * plus an independent FP64 dense-grid Amanatides-Woo brute force (orc_bruteforce)
  * that pins the hit voxel and t without using the octree at all.
  */
@@ -51,6 +54,7 @@ typedef struct orc_scene {
     float grid_max[3];
     int32_t max_levels;         /* log2(N) */
     int32_t bricks_per_axis;    /* N / 8 */
+    const uint64_t* normals;    /* brick_count x 32 DXT normal blocks (2 u64 each), or NULL */
 } orc_scene;
 
 /* counts[0] rays, [1] ESVO iterations, [2] brick visits, [3] DDA voxels, [4] hits */
@@ -66,6 +70,23 @@ void orc_camera_rays(const float cam[16], uint32_t w, uint32_t h, float tmax, or
 
 /* BC1 decode of one texel. */
 void orc_bc1_decode(uint64_t block, int texel, float rgb[3]);
+
+/* decodeDXT1Color as the compressed shader evaluates it (Compression.glsl:60-75). */
+void orc_bc1_decode_shader(uint64_t block, int texel, float rgb[3]);
+
+/* decodeDXTNormal (Compression.glsl:115-177), unnormalised. */
+void orc_dxt_normal_decode(uint64_t block_a, uint64_t block_b, int texel, float n[3]);
+
+/* One pixel of vx_render's RGBA8 image (R in bits 0-7, A = 255) for ray r, its
+ * hit record h and the pixel's v = (py + 0.5) / height: sky on a miss; on a
+ * hit computeLighting with the compressed shader's colour and normal when the
+ * scene has BC1 colours and normals, else the record's colour and entry-face
+ * normal. */
+uint32_t orc_shade(const orc_scene* s, const orc_ray* r, const orc_hit* h, float v);
+
+/* vx_render's RGBA8 image: camera rays, traversal and orc_shade, w*h row-major. */
+void orc_render(const orc_scene* s, const float cam[16], uint32_t w, uint32_t h, float tmax, uint32_t* rgba,
+                int nthreads);
 
 /* RGBA8 colour of an uncompressed voxel word. */
 void orc_rgba8_decode(uint32_t word, float rgb[3]);

@@ -49,6 +49,8 @@ _Static_assert(offsetof(vx_octree_config, grid_min) == 32 && offsetof(vx_octree_
                offsetof(vx_octree_config, world_to_local) == 128 &&
                offsetof(vx_octree_config, _padding4) == 192, "config offsets");
 _Static_assert(offsetof(vx_hit, rgb) == 20 && offsetof(vx_ray, d) == 16, "record offsets");
+_Static_assert(offsetof(vx_scene_desc, config) == 40 && offsetof(vx_scene_desc, dxt_normals) == 48 &&
+               sizeof(vx_scene_desc) == 56, "scene descriptor");
 int main(void) { return 0; }
 """)
     r = subprocess.run(["gcc", "-std=c11", "-fsyntax-only", f"-I{ROOT / 'include'}", str(src)],
@@ -60,6 +62,7 @@ def test_numpy_dtypes_match_abi(vx):
     assert vx.RAY_DTYPE.itemsize == C.sizeof(C.c_float) * 8
     assert vx.HIT_DTYPE.fields["rgb"][1] == 20
     assert C.sizeof(vx.VxOctreeConfig) == 256 and C.sizeof(vx.VxCamera) == 64
+    assert C.sizeof(vx.VxSceneDesc) == 56 and vx.VxSceneDesc.dxt_normals.offset == 48
 
 
 def test_version_and_no_device_errors(vx):

@@ -149,17 +149,35 @@ def test_far_pointer_scene_bit_exact(vx, orc, gpu_ctx):
     assert (grid[g[hit, 2], g[hit, 1], g[hit, 0]] != 0).all()
 
 
-def test_render_rgba_image(vx, cornell_bc1, gpu_ctx):
-    grid, cam, built = cornell_bc1
+def _rgba_vs_oracle(vx, orc, gpu_ctx, built, cam, w, h, tmax, label):
     scene = vx.Scene(gpu_ctx, built)
-    hits, img = scene.render(cam, 128, 128, tmax=1e4, hits=True, rgba=True)
+    _, img = scene.render(cam, w, h, tmax=tmax, hits=False, rgba=True)
     _sync()
-    h = vx.hits_to_numpy(hits)
-    px = img.cpu().numpy().view(np.uint32)
-    assert ((px >> 24) == 255).all()
-    hit = (h["flags"] & 1) == 1
-    # shaded hits are the decoded colour times (0.3 + 0.7 max(n.l, 0)), in [0.3, 1] of it
-    r = (px & 255).astype(np.float64) / 255.0
-    want = h["rgb"][:, 0].astype(np.float64)
-    assert np.all(r[hit] <= want[hit] + 1.0 / 255 + 1e-6)
-    assert np.all(r[hit] >= 0.3 * want[hit] - 1.0 / 255 - 1e-6)
+    got = img.cpu().numpy().view(np.uint32)
+    want = orc.render(orc.OracleScene.from_built(built), cam.as_floats(), w, h, tmax, nthreads=16)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} of {w * h} RGBA8 pixels differ, first {bad[:4]}"
+    assert ((got >> 24) == 255).all()
+    return got
+
+
+@pytest.mark.parametrize("scene_name", ["cornell", "cornell_bc1"])
+def test_render_rgba_entry_face_shading_bit_exact(vx, orc, scene_name, request, gpu_ctx):
+    # no normal pool: computeLighting with the hit record's colour and entry-face normal
+    grid, cam, built = request.getfixturevalue(scene_name)
+    _rgba_vs_oracle(vx, orc, gpu_ctx, built, cam, 512, 512, 1e4, scene_name)
+
+
+def test_render_rgba_compressed_shading_bit_exact(vx, orc, cornell_bc1, cave, gpu_ctx):
+    # BC1 colours + DXT normals: the compressed shader's colour decode and normal blend
+    grid, cam, built = cornell_bc1
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    _rgba_vs_oracle(vx, orc, gpu_ctx, b, cam, 512, 512, 1e4, "cornell+normals")
+    grid, cam, built = cave
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    img = _rgba_vs_oracle(vx, orc, gpu_ctx, b, cam, 1920, 1080, 1e30, "cave+normals")
+    # the normal pool changes the shading of some hit pixels
+    face = _rgba_vs_oracle(vx, orc, gpu_ctx, built, cam, 1920, 1080, 1e30, "cave")
+    assert (img != face).mean() > 0.05

@@ -76,7 +76,7 @@ class VxCamera(C.Structure):
 class VxSceneDesc(C.Structure):
     _fields_ = [("esvo_nodes", C.c_void_p), ("node_count", C.c_uint64),
                 ("brick_data", C.c_void_p), ("brick_count", C.c_uint64),
-                ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig))]
+                ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig)), ("dxt_normals", C.c_void_p)]
 
 
 class VxTuning(C.Structure):
@@ -131,6 +131,9 @@ def _load() -> C.CDLL:
         "vx_build_from_dense": ([vp, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
+        "vx_encode_normals": ([vp, u64, C.c_int, vp], st),
+        "vx_dxt_normal_encode_block": ([vp, vp], None),
+        "vx_geometric_normal": ([vp, C.c_int, C.c_int, C.c_int, vp], None),
         "vx_scene_set_lds_depth": ([vp, u32], st),
         "vx_batch_create": ([vp, C.POINTER(vp), u32, C.POINTER(vp)], st),
         "vx_batch_destroy": ([vp], st),
@@ -161,7 +164,8 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
-            "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays")
+            "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
+            "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal")
 
 
 def _check(status: int, where: str) -> None:
@@ -185,6 +189,7 @@ class BuiltScene:
     brick_coords: np.ndarray  # (brick_count, 3) int32 grid cell per brick id
     internal_count: int
     config: VxOctreeConfig
+    normals: np.ndarray | None = None  # (brick_count, 64) uint64 DXT normal blocks, see encode_normals
 
     @property
     def N(self) -> int:
@@ -200,7 +205,17 @@ class BuiltScene:
 
     def device_bytes(self) -> int:
         nb = self.brick_count
-        return self.node_count * 8 + nb * 2048 + nb * 64 + (nb * 256 if self.bc1 is not None else 0)
+        return (self.node_count * 8 + nb * 2048 + nb * 64 + (nb * 256 if self.bc1 is not None else 0) +
+                (nb * 512 if self.normals is not None else 0))
+
+    def encode_normals(self, nthreads: int = 0) -> "BuiltScene":
+        """Adds the compressed shader's normal pool (vx_encode_normals) and returns self."""
+        out = np.zeros((self.brick_count, 64), np.uint64)
+        if self.brick_count:
+            _check(LIB.vx_encode_normals(_ptr(self.brick_data), self.brick_count, nthreads, _ptr(out)),
+                   "vx_encode_normals")
+        self.normals = out
+        return self
 
 
 def make_camera(pos, look_dir, fov_deg: float, aspect: float) -> VxCamera:
@@ -264,6 +279,22 @@ def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0) -> B
 def bc1_encode_block(colors: np.ndarray) -> int:
     c = np.ascontiguousarray(colors, dtype=np.float32).reshape(16, 3)
     return int(LIB.vx_bc1_encode_block(_ptr(c)))
+
+
+def dxt_normal_encode_block(normals: np.ndarray) -> tuple[int, int]:
+    """DXTNormalCompressor::encodeBlockTyped of 16 normals: (blockA, blockB)."""
+    n = np.ascontiguousarray(normals, dtype=np.float32).reshape(16, 3)
+    out = np.zeros(2, np.uint64)
+    LIB.vx_dxt_normal_encode_block(_ptr(n), _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def geometric_normal(brick: np.ndarray, x: int, y: int, z: int) -> np.ndarray:
+    """computeGeometricNormal of voxel (x, y, z) in one brick of 512 words."""
+    b = np.ascontiguousarray(brick, dtype=np.uint32).reshape(512)
+    out = np.zeros(3, np.float32)
+    LIB.vx_geometric_normal(_ptr(b), x, y, z, _ptr(out))
+    return out
 
 
 def _gen(fn, where, N, *args):
@@ -335,6 +366,7 @@ class Scene:
         desc.brick_data = _ptr(built.brick_data) if built.brick_count else None
         desc.brick_count = built.brick_count
         desc.bc1_colors = _ptr(built.bc1) if (built.bc1 is not None and built.brick_count) else None
+        desc.dxt_normals = _ptr(built.normals) if (built.normals is not None and built.brick_count) else None
         self._cfg = built.config
         desc.config = C.pointer(self._cfg)
         h = C.c_void_p()

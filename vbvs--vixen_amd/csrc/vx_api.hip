@@ -132,8 +132,105 @@ __device__ __forceinline__ uint32_t to_u8(float x)
     return (uint32_t)(x * 255.0f + 0.5f);
 }
 
+// computeLighting's fixed light, normalize(vec3(1, 1, -1)), folded to the
+// correctly rounded 1/sqrt(3) as a compiler folds the constant.
+constexpr float kLight = 0.57735026918962576f;
+
+// decodeDXT1Color as the compressed shader evaluates it (shaders/Compression.glsl:28-40,60-75):
+// c0 from {2^-32, 0, 2*2^-32/3, 2^-32/3}, c1 = 2^-32 - c0, channel = c0 * float(head << k) + c1 * float(head << m).
+__device__ __forceinline__ void bc1_decode_shader(uint2 block, uint32_t texel, float rgb[3])
+{
+    const float S = 0x1p-32f;
+    const uint32_t sel = (block.y >> (2u * texel)) & 3u;
+    const float c0 = sel == 0u ? S : (sel == 1u ? 0.0f : (sel == 2u ? (2.0f * S) / 3.0f : (1.0f * S) / 3.0f));
+    const float c1 = S - c0;
+    const uint32_t h = block.x;
+    rgb[0] = c0 * (float)(h << 16) + c1 * (float)h;
+    rgb[1] = c0 * (float)(h << 21) + c1 * (float)(h << 5);
+    rgb[2] = c0 * (float)(h << 27) + c1 * (float)(h << 11);
+}
+
+// decodeDXTNormal (shaders/Compression.glsl:115-177): raw base normal plus the
+// U and V axes weighted by {-1, -1/3, 1/3, 1}. Unnormalised.
+__device__ __forceinline__ void dxt_normal_decode(uint4 blk, uint32_t texel, float n[3])
+{
+    const uint32_t hb = blk.x, bits_u = blk.y, huv = blk.z, bits_v = blk.w;
+    const int32_t sgn = (int32_t)hb >> 31;
+    const uint32_t axis = (hb >> 29) & 3u;
+    const float t = (float)(sgn ^ 0x7FFFFFFF), u = (float)(int32_t)(hb << 3), v = (float)(int32_t)(hb << 18);
+    const float bx = axis == 0u ? t : (axis == 1u ? v : u);
+    const float by = axis == 0u ? u : (axis == 1u ? t : v);
+    const float bz = axis == 0u ? v : (axis == 1u ? u : t);
+    float ax[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int32_t sh = (int32_t)(huv << (16 - 16 * k));
+        const int e = (sh >> 16) & 15;
+        const float scale = __uint_as_float((uint32_t)(e + (3 - 13) + 127) << 23); // exp2(e - 10), e in [0,15]
+        ax[k][0] = (float)sh * scale;
+        ax[k][1] = (float)(int32_t)((uint32_t)sh << 4) * scale;
+        ax[k][2] = (float)(int32_t)((uint32_t)sh << 8) * scale;
+    }
+    const uint32_t su = (bits_u >> (2u * texel)) & 3u, sv = (bits_v >> (2u * texel)) & 3u;
+    const float c13 = 1.0f / 3.0f;
+    const float cu = su == 0u ? -1.0f : (su == 1u ? -c13 : (su == 2u ? c13 : 1.0f));
+    const float cv = sv == 0u ? -1.0f : (sv == 1u ? -c13 : (sv == 2u ? c13 : 1.0f));
+    n[0] = (bx + ax[0][0] * cu) + ax[1][0] * cv;
+    n[1] = (by + ax[0][1] * cu) + ax[1][1] * cv;
+    n[2] = (bz + ax[0][2] * cu) + ax[1][2] * cv;
+}
+
+__device__ __forceinline__ void normalize3(float v[3])
+{
+    const float inv = 1.0f / __builtin_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] *= inv;
+    v[1] *= inv;
+    v[2] *= inv;
+}
+
+// The compressed shader's shading of a hit (VoxelRayMarch_Compressed.comp:154-165,
+// 239-256,541-542; shaders/Lighting.glsl:19-28): colour from the shader's BC1
+// decode, normal from the DXT normal block blended with the DDA step normal, or
+// the ray's dominant-axis normal when the DDA took no step in the hit brick.
+// du is the normalised world direction.
+__device__ __forceinline__ void shade_compressed(const SceneView& s, const vx_hit& h, const float du[3],
+                                                 float col[3])
+{
+    float rgb[3];
+    bc1_decode_shader(s.bc1[(size_t)h.brick * 32u + (h.voxel >> 4)], h.voxel & 15u, rgb);
+    const uint32_t last = (h.flags >> 4) & 3u;
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    if (last == 0u) {
+        const float a0 = __builtin_fabsf(du[0]), a1 = __builtin_fabsf(du[1]), a2 = __builtin_fabsf(du[2]);
+        const int ax = (a0 > a1 && a0 > a2) ? 0 : (a1 > a2 ? 1 : 2);
+        const float d = ax == 0 ? du[0] : (ax == 1 ? du[1] : du[2]);
+        const float sg = d > 0.0f ? -1.0f : (d < 0.0f ? 1.0f : 0.0f); // -sign(d)
+        n[0] = ax == 0 ? sg : 0.0f;
+        n[1] = ax == 1 ? sg : 0.0f;
+        n[2] = ax == 2 ? sg : 0.0f;
+    } else {
+        dxt_normal_decode(s.nrm[(size_t)h.brick * 32u + (h.voxel >> 4)], h.voxel & 15u, n);
+        normalize3(n);
+        const float d = last == 1u ? du[0] : (last == 2u ? du[1] : du[2]);
+        const float dda = d < 0.0f ? 1.0f : -1.0f; // -step, step = sign(d) with 0 -> 1
+        n[0] = n[0] + (last == 1u ? dda : 0.0f) * 0.5f;
+        n[1] = n[1] + (last == 2u ? dda : 0.0f) * 0.5f;
+        n[2] = n[2] + (last == 3u ? dda : 0.0f) * 0.5f;
+        normalize3(n);
+    }
+    const float dt = (n[0] * kLight + n[1] * kLight) + n[2] * -kLight;
+    const float k = 0.3f + (dt < 0.0f ? 0.0f : dt) * 0.7f;
+    col[0] = rgb[0] * k;
+    col[1] = rgb[1] * k;
+    col[2] = rgb[2] * k;
+}
+
+// Shading of vx_render's RGBA8 image, chosen per launch so the hits-only
+// kernel carries none of it.
+enum : int { SHADE_NONE = 0, SHADE_FACE = 1, SHADE_COMPRESSED = 2 };
+
 // 16x16 pixel tile per workgroup; each wave covers an 8x8 sub-tile.
-template <bool STAGED>
+template <bool STAGED, int SHADE>
 __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uint32_t w, uint32_t h, float tmax,
                                                    uint32_t tiles_x, uint32_t ntiles, uint32_t per_xcd,
                                                    vx_hit* __restrict__ hits, uint32_t* __restrict__ rgba)
@@ -152,9 +249,15 @@ __global__ void __launch_bounds__(kBlock) k_render(SceneView s, CamParams c, uin
     const bool hit = trace_ray<false, STAGED, false>(s, c.pos, d, 0.0f, tmax, hr, cnt);
     const uint64_t i = (uint64_t)py * w + px;
     if (hits) store_hit(hits, i, hr);
-    if (rgba) {
+    if (SHADE != SHADE_NONE) {
         float col[3];
-        if (hit) {
+        if (SHADE == SHADE_COMPRESSED && hit) {
+            float du[3];
+            const float inv = 1.0f / __builtin_sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) du[i] = d[i] * inv; // as setup_ray normalises it
+            shade_compressed(s, hr, du, col);
+        } else if (hit) {
             // computeLighting (shaders/Lighting.glsl:19-28) with the entry-face normal
             const uint32_t axis = (hr.flags >> 1) & 3u;
             const float sgn = (hr.flags & 8u) ? 1.0f : -1.0f;
@@ -364,18 +467,22 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
         (void)hipFree(s->brick_data);
         (void)hipFree(s->occ);
         (void)hipFree(s->bc1);
+        (void)hipFree(s->nrm);
         delete s;
         return st;
     };
     const uint64_t nb = d->brick_count;
     const size_t node_bytes = d->node_count * 8, brick_bytes = std::max<uint64_t>(1, nb) * 2048,
-                 occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256;
+                 occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256,
+                 nrm_bytes = std::max<uint64_t>(1, nb) * 512;
     if (hipMalloc(&s->nodes, node_bytes) != hipSuccess || hipMalloc(&s->brick_data, brick_bytes) != hipSuccess ||
-        hipMalloc(&s->occ, occ_bytes) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess))
+        hipMalloc(&s->occ, occ_bytes) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess) ||
+        (d->dxt_normals && hipMalloc(&s->nrm, nrm_bytes) != hipSuccess))
         return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
     if (hipMemcpy(s->nodes, d->esvo_nodes, node_bytes, hipMemcpyHostToDevice) != hipSuccess ||
         (nb && hipMemcpy(s->brick_data, d->brick_data, nb * 2048, hipMemcpyHostToDevice) != hipSuccess) ||
-        (nb && d->bc1_colors && hipMemcpy(s->bc1, d->bc1_colors, nb * 256, hipMemcpyHostToDevice) != hipSuccess))
+        (nb && d->bc1_colors && hipMemcpy(s->bc1, d->bc1_colors, nb * 256, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nb && d->dxt_normals && hipMemcpy(s->nrm, d->dxt_normals, nb * 512, hipMemcpyHostToDevice) != hipSuccess))
         return cleanup(fail(VX_ERR_HIP, "vx_scene_create: upload failed"));
     if (nb) {
         const uint64_t words = nb * 16;
@@ -388,6 +495,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.brick_data = s->brick_data;
     v.occ = s->occ;
     v.bc1 = d->bc1_colors ? s->bc1 : nullptr;
+    v.nrm = d->dxt_normals ? s->nrm : nullptr;
     v.node_count = static_cast<uint32_t>(d->node_count);
     v.brick_count = static_cast<uint32_t>(nb);
     const uint32_t N = B * 8;
@@ -507,6 +615,7 @@ vx_status vx_scene_destroy(vx_scene* s)
     (void)hipFree(s->brick_data);
     (void)hipFree(s->occ);
     (void)hipFree(s->bc1);
+    (void)hipFree(s->nrm);
     delete s;
     return VX_OK;
 }
@@ -554,10 +663,19 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     uint32_t lds = 0;
     const vx::SceneView v = vx::launch_view(s, lds);
     const dim3 grid(per * vx::kXcds), block(vx::kBlock);
-    if (v.lds_count > 0)
-        vx::k_render<true><<<grid, block, lds, st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
-    else
-        vx::k_render<false><<<grid, block, lds, st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
+    const int shade = !rgba8 ? vx::SHADE_NONE : ((v.nrm && v.bc1) ? vx::SHADE_COMPRESSED : vx::SHADE_FACE);
+    auto launch = [&](auto kernel) {
+        kernel<<<grid, block, lds, st>>>(v, c, w, h, tmax, tiles_x, ntiles, per, hits, rgba8);
+    };
+    if (v.lds_count > 0) {
+        if (shade == vx::SHADE_NONE) launch(vx::k_render<true, vx::SHADE_NONE>);
+        else if (shade == vx::SHADE_FACE) launch(vx::k_render<true, vx::SHADE_FACE>);
+        else launch(vx::k_render<true, vx::SHADE_COMPRESSED>);
+    } else {
+        if (shade == vx::SHADE_NONE) launch(vx::k_render<false, vx::SHADE_NONE>);
+        else if (shade == vx::SHADE_FACE) launch(vx::k_render<false, vx::SHADE_FACE>);
+        else launch(vx::k_render<false, vx::SHADE_COMPRESSED>);
+    }
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_render: launch failed");
     return VX_OK;
 }

@@ -318,6 +318,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         v.brick_data = b->brick_data + bb * 512;
         v.occ = b->occ + bb * 16;
         v.bc1 = sc->view.bc1 ? b->bc1 + cb * 32 : nullptr;
+        v.nrm = nullptr; // the batched launch writes hit records only (no shading)
         v.lds_off = 0;
         v.lds_count = 0;
         b->views.push_back(v);

@@ -25,6 +25,7 @@ struct vx_scene {
     uint32_t* brick_data;
     uint32_t* occ;
     uint2* bc1;
+    uint4* nrm; // DXT normal blocks (32 per brick) or nullptr
     uint64_t node_count;
     uint64_t brick_count;
     vx_octree_config cfg;

@@ -54,6 +54,7 @@ struct SceneView {
     const uint32_t* __restrict__ brick_data;  // 512 words per brick (colour when no BC1)
     const uint32_t* __restrict__ occ;         // 16 words per brick: occupancy bits
     const uint2* __restrict__ bc1;            // 32 blocks per brick or nullptr
+    const uint4* __restrict__ nrm;            // 32 DXT normal blocks per brick or nullptr
     uint32_t node_count;
     uint32_t brick_count;
     float gmin[3];
@@ -239,6 +240,7 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
     uint32_t word = occ[lin >> 5];
     bool hit = false;
+    uint32_t last = 0u; // axis of the step into the current voxel (1 x, 2 y, 3 z; 0 = entry voxel)
     for (int it = 0;; ++it) {
         if (COUNT) cnt.voxels += 1;
         // bitwise & on the conditions: no short-circuit branches
@@ -266,6 +268,7 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         cur[2] = n2;
         lin = nlin;
         word = nword;
+        last = ax ? 1u : (ay ? 2u : 3u);
     }
     if (!hit) return false;
 
@@ -296,7 +299,7 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         axis = 2u;
         signpos = sz;
     }
-    h.flags = 1u | (axis << 1) | (signpos << 3);
+    h.flags = 1u | (axis << 1) | (signpos << 3) | (last << 4);
     h.t = tHit;
     h.brick = brick;
     h.voxel = lin;
