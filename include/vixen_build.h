@@ -70,6 +70,65 @@ void vx_dxt_normal_encode_block(const float normals[16][3], uint64_t out[2]);
  * of 512 words: -normalize(occupancy gradient), or (0,1,0) when it vanishes. */
 void vx_geometric_normal(const uint32_t brick[512], int x, int y, int z, float out[3]);
 
+/* ------------------------------------------------------------------------ */
+/* VIXEN's on-disk scene cache, VSCA version 1                                */
+/* (VoxelSceneCacher::SerializeToFile / DeserializeFromFile,                  */
+/*  libraries/CashSystem/src/VoxelSceneCacher.cpp:179-364)                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vx_cache vx_cache;
+
+/* What an entry holds besides the scene buffers (VoxelSceneCreateInfo and the
+ * metadata of VoxelSceneData, VoxelSceneCacher.h:145-173,222-227). materials
+ * are material_count GPUMaterial records of 32 bytes, brick_grid_lookup maps
+ * bx + B*by + B*B*bz to a brick id (~0u when empty); both are carried as they
+ * are (the ray-cast path reads neither). Pointers stay valid until
+ * vx_cache_free. */
+typedef struct vx_cache_entry_info {
+    uint64_t key;                /* ((sceneType*31 + resolution)*31 + (u32)(density*100))*31 + seed */
+    uint32_t scene_type;         /* 0 cornell, 1 noise, 2 tunnels, 3 cityscape, 255 custom */
+    uint32_t resolution;
+    float density;
+    uint32_t seed;
+    uint32_t node_count;
+    uint32_t brick_count;
+    uint32_t solid_voxel_count;
+    const void* materials;
+    uint64_t material_count;
+    const uint32_t* brick_grid_lookup;
+    uint64_t brick_grid_lookup_count;
+} vx_cache_entry_info;
+
+/* Reads a cache file. Errors follow the reference loader's cases: no file ->
+ * VX_ERR_INVALID_ARG; bad magic, a key that does not match its create info,
+ * a truncated file or buffer sizes that disagree with the node/brick counts ->
+ * VX_ERR_INVALID_SCENE; another version -> VX_ERR_UNSUPPORTED (the reference
+ * treats it as stale and regenerates). */
+vx_status vx_cache_read(const char* path, vx_cache** out);
+vx_status vx_cache_entry_count(const vx_cache* cache, uint32_t* count);
+/* Entry index: its info and a vx_scene_desc pointing into the cache (ready for
+ * vx_scene_create; bc1_colors / dxt_normals are NULL when the entry has none).
+ * Either output may be NULL. */
+vx_status vx_cache_entry(const vx_cache* cache, uint32_t index, vx_cache_entry_info* info, vx_scene_desc* desc);
+vx_status vx_cache_free(vx_cache* cache);
+
+/* One entry to write: create info, the scene buffers and the carried extras. */
+typedef struct vx_cache_entry_in {
+    uint32_t scene_type;
+    uint32_t resolution;
+    float density;
+    uint32_t seed;
+    const vx_scene_desc* scene;
+    const void* materials;       /* material_count x 32 bytes, or NULL */
+    uint64_t material_count;
+    const uint32_t* brick_grid_lookup; /* or NULL */
+    uint64_t brick_grid_lookup_count;
+    uint32_t solid_voxel_count;
+} vx_cache_entry_in;
+
+/* Writes a VSCA v1 file VIXEN's VoxelSceneCacher::DeserializeFromFile loads. */
+vx_status vx_cache_write(const char* path, const vx_cache_entry_in* entries, uint32_t n);
+
 /* Synthetic scenes (voxel units, world = [0,N]^3). Each fills grid (N^3 words)
  * and a camera (vx_camera, fov in degrees) for the scene's primary view.
  *   cornell: N^3 room, 1-voxel walls x=0 red, x=N-1 green, y=0, y=N-1 and

@@ -181,3 +181,18 @@ def test_render_rgba_compressed_shading_bit_exact(vx, orc, cornell_bc1, cave, gp
     # the normal pool changes the shading of some hit pixels
     face = _rgba_vs_oracle(vx, orc, gpu_ctx, built, cam, 1920, 1080, 1e30, "cave")
     assert (img != face).mean() > 0.05
+
+
+def test_scene_from_cache_file_renders_like_built(vx, orc, cave, gpu_ctx, tmp_path):
+    # a VSCA v1 cache round trip (VIXEN's scene cache format) feeds vx_scene_create unchanged
+    grid, cam, built = cave
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    f = tmp_path / "cave.vsca"
+    vx.write_cache(f, [dict(built=b, scene_type="tunnels", resolution=256, density=0.5, seed=2)])
+    (_, loaded), = vx.read_cache(f)
+    h1, i1 = vx.Scene(gpu_ctx, b).render(cam, 640, 360, tmax=1e30, rgba=True)
+    h2, i2 = vx.Scene(gpu_ctx, loaded).render(cam, 640, 360, tmax=1e30, rgba=True)
+    _sync()
+    assert vx.hits_to_numpy(h1).tobytes() == vx.hits_to_numpy(h2).tobytes()
+    assert i1.cpu().numpy().tobytes() == i2.cpu().numpy().tobytes()

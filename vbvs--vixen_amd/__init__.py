@@ -79,6 +79,20 @@ class VxSceneDesc(C.Structure):
                 ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig)), ("dxt_normals", C.c_void_p)]
 
 
+class VxCacheEntryInfo(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("scene_type", C.c_uint32), ("resolution", C.c_uint32), ("density", C.c_float),
+                ("seed", C.c_uint32), ("node_count", C.c_uint32), ("brick_count", C.c_uint32),
+                ("solid_voxel_count", C.c_uint32), ("materials", C.c_void_p), ("material_count", C.c_uint64),
+                ("brick_grid_lookup", C.c_void_p), ("brick_grid_lookup_count", C.c_uint64)]
+
+
+class VxCacheEntryIn(C.Structure):
+    _fields_ = [("scene_type", C.c_uint32), ("resolution", C.c_uint32), ("density", C.c_float), ("seed", C.c_uint32),
+                ("scene", C.POINTER(VxSceneDesc)), ("materials", C.c_void_p), ("material_count", C.c_uint64),
+                ("brick_grid_lookup", C.c_void_p), ("brick_grid_lookup_count", C.c_uint64),
+                ("solid_voxel_count", C.c_uint32)]
+
+
 class VxTuning(C.Structure):
     _fields_ = [("waves_per_group", C.c_uint32), ("chunk", C.c_uint32), ("lds_depth", C.c_uint32),
                 ("refill_min", C.c_uint32)]
@@ -132,6 +146,11 @@ def _load() -> C.CDLL:
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
         "vx_encode_normals": ([vp, u64, C.c_int, vp], st),
+        "vx_cache_read": ([C.c_char_p, C.POINTER(vp)], st),
+        "vx_cache_entry_count": ([vp, C.POINTER(u32)], st),
+        "vx_cache_entry": ([vp, u32, C.POINTER(VxCacheEntryInfo), C.POINTER(VxSceneDesc)], st),
+        "vx_cache_free": ([vp], st),
+        "vx_cache_write": ([C.c_char_p, C.POINTER(VxCacheEntryIn), u32], st),
         "vx_dxt_normal_encode_block": ([vp, vp], None),
         "vx_geometric_normal": ([vp, C.c_int, C.c_int, C.c_int, vp], None),
         "vx_scene_set_lds_depth": ([vp, u32], st),
@@ -165,7 +184,8 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
             "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
-            "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal")
+            "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
+            "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write")
 
 
 def _check(status: int, where: str) -> None:
@@ -289,6 +309,104 @@ def dxt_normal_encode_block(normals: np.ndarray) -> tuple[int, int]:
     return int(out[0]), int(out[1])
 
 
+def _scene_desc(built: BuiltScene) -> tuple[VxSceneDesc, VxOctreeConfig]:
+    desc = VxSceneDesc()
+    desc.esvo_nodes = _ptr(built.nodes)
+    desc.node_count = built.node_count
+    desc.brick_data = _ptr(built.brick_data) if built.brick_count else None
+    desc.brick_count = built.brick_count
+    desc.bc1_colors = _ptr(built.bc1) if (built.bc1 is not None and built.brick_count) else None
+    desc.dxt_normals = _ptr(built.normals) if (built.normals is not None and built.brick_count) else None
+    cfg = built.config
+    desc.config = C.pointer(cfg)
+    return desc, cfg
+
+
+def brick_grid_lookup(built: BuiltScene) -> np.ndarray:
+    """VIXEN's brickGridLookup: bx + B*by + B*B*bz -> brick id, ~0u where no brick."""
+    B = int(built.config.bricks_per_axis)
+    out = np.full(B * B * B, 0xFFFFFFFF, np.uint32)
+    c = built.brick_coords.astype(np.int64)
+    out[c[:, 0] + B * c[:, 1] + B * B * c[:, 2]] = np.arange(built.brick_count, dtype=np.uint32)
+    return out
+
+
+SCENE_TYPES = {"cornell": 0, "noise": 1, "tunnels": 2, "cityscape": 3, "custom": 255}
+
+
+def write_cache(path, entries) -> None:
+    """Writes a VSCA v1 scene cache. entries: dicts with built (BuiltScene), scene_type (int or name),
+    resolution, density, seed, and optionally materials (bytes/uint8 array of 32-byte records),
+    solid_voxel_count; the brick grid lookup is derived from built.brick_coords."""
+    keep, ins = [], (VxCacheEntryIn * max(1, len(entries)))()
+    for i, e in enumerate(entries):
+        built = e["built"]
+        desc, cfg = _scene_desc(built)
+        lookup = brick_grid_lookup(built)
+        mats = np.frombuffer(bytes(e.get("materials", b"")), np.uint8)
+        st = e.get("scene_type", 255)
+        ins[i].scene_type = SCENE_TYPES[st] if isinstance(st, str) else int(st)
+        ins[i].resolution = int(e.get("resolution", built.N))
+        ins[i].density = float(e.get("density", 0.5))
+        ins[i].seed = int(e.get("seed", 0))
+        ins[i].scene = C.pointer(desc)
+        ins[i].materials = _ptr(mats) if mats.size else None
+        ins[i].material_count = mats.size // 32
+        ins[i].brick_grid_lookup = _ptr(lookup)
+        ins[i].brick_grid_lookup_count = lookup.size
+        ins[i].solid_voxel_count = int(e.get("solid_voxel_count", int((built.brick_data != 0).sum())))
+        keep.append((desc, cfg, lookup, mats, built))
+    _check(LIB.vx_cache_write(str(path).encode(), ins, len(entries)), "vx_cache_write")
+
+
+def read_cache(path) -> list[tuple[dict, BuiltScene]]:
+    """Reads a VSCA v1 scene cache: per entry (info dict, BuiltScene with copies of its buffers)."""
+    h = C.c_void_p()
+    _check(LIB.vx_cache_read(str(path).encode(), C.byref(h)), "vx_cache_read")
+    try:
+        n = C.c_uint32()
+        _check(LIB.vx_cache_entry_count(h, C.byref(n)), "vx_cache_entry_count")
+        out = []
+        for i in range(n.value):
+            info, desc = VxCacheEntryInfo(), VxSceneDesc()
+            _check(LIB.vx_cache_entry(h, i, C.byref(info), C.byref(desc)), "vx_cache_entry")
+            nn, nb = int(desc.node_count), int(desc.brick_count)
+
+            def arr(ptr, count, dtype):
+                if not ptr or count == 0:
+                    return None
+                return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                             (count,)).copy()
+            nodes = arr(desc.esvo_nodes, nn * 2, np.uint32).reshape(nn, 2)
+            bricks = arr(desc.brick_data, nb * 512, np.uint32)
+            bricks = bricks.reshape(nb, 512) if bricks is not None else np.zeros((0, 512), np.uint32)
+            bc1 = arr(desc.bc1_colors, nb * 32, np.uint64)
+            nrm = arr(desc.dxt_normals, nb * 64, np.uint64)
+            lookup = arr(info.brick_grid_lookup, int(info.brick_grid_lookup_count), np.uint32)
+            cfg = VxOctreeConfig.from_buffer_copy(bytes(desc.config.contents))
+            B = int(cfg.bricks_per_axis)
+            coords = np.zeros((nb, 3), np.int32)
+            if lookup is not None and lookup.size == B ** 3:
+                cells = np.nonzero(lookup != 0xFFFFFFFF)[0]
+                ids = lookup[cells]
+                ok = ids < nb
+                coords[ids[ok], 0] = cells[ok] % B
+                coords[ids[ok], 1] = (cells[ok] // B) % B
+                coords[ids[ok], 2] = cells[ok] // (B * B)
+            mats = arr(info.materials, int(info.material_count) * 32, np.uint8)
+            meta = {"key": int(info.key), "scene_type": int(info.scene_type), "resolution": int(info.resolution),
+                    "density": float(info.density), "seed": int(info.seed), "node_count": int(info.node_count),
+                    "brick_count": int(info.brick_count), "solid_voxel_count": int(info.solid_voxel_count),
+                    "materials": b"" if mats is None else mats.tobytes(),
+                    "brick_grid_lookup": lookup}
+            built = BuiltScene(nodes, bricks, None if bc1 is None else bc1.reshape(nb, 32), coords,
+                               max(0, nn - nb), cfg, None if nrm is None else nrm.reshape(nb, 64))
+            out.append((meta, built))
+        return out
+    finally:
+        LIB.vx_cache_free(h)
+
+
 def geometric_normal(brick: np.ndarray, x: int, y: int, z: int) -> np.ndarray:
     """computeGeometricNormal of voxel (x, y, z) in one brick of 512 words."""
     b = np.ascontiguousarray(brick, dtype=np.uint32).reshape(512)
@@ -360,15 +478,7 @@ class Scene:
     def __init__(self, ctx: Context, built: BuiltScene):
         self.ctx = ctx
         self.built = built
-        desc = VxSceneDesc()
-        desc.esvo_nodes = _ptr(built.nodes)
-        desc.node_count = built.node_count
-        desc.brick_data = _ptr(built.brick_data) if built.brick_count else None
-        desc.brick_count = built.brick_count
-        desc.bc1_colors = _ptr(built.bc1) if (built.bc1 is not None and built.brick_count) else None
-        desc.dxt_normals = _ptr(built.normals) if (built.normals is not None and built.brick_count) else None
-        self._cfg = built.config
-        desc.config = C.pointer(self._cfg)
+        desc, self._cfg = _scene_desc(built)
         h = C.c_void_p()
         _check(LIB.vx_scene_create(ctx._h, C.byref(desc), C.byref(h)), "vx_scene_create")
         self._h = h

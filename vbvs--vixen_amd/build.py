@@ -38,7 +38,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp"]
+HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp", "vx_cache.cpp"]
 HIP_SOURCES = ["vx_api.hip", "vx_batch.hip", "vx_albedo.hip"]
 
 
