@@ -28,6 +28,11 @@ Also:
   cpu_baseline  the strict-FP32 C oracle on the host cores, same frame, all
                 cores (median of 5 after a warm-up): reported, not a target
   parity        the last timed frame compared bit for bit with the oracle's
+
+--vixen-json PATH also writes the headline frames in the per-test results schema
+of VIXEN's benchmark runner (libraries/Profiler/src/MetricsExporter.cpp:117-230),
+pipeline "hip", so its results tooling can chart this backend beside the Vulkan
+pipelines.
 """
 from __future__ import annotations
 
@@ -257,6 +262,36 @@ def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int):
             "bounce_mrays": round(W * H / ms_bounce / 1e3, 1)}
 
 
+# ---------------------------------------------------------------------------
+# VIXEN results schema
+# ---------------------------------------------------------------------------
+def vixen_report(frame_ms, rays, read_bytes, write_bytes, iters_per_ray, voxel_res, scene_type, width, height,
+                 gpu_name, driver, vram_total_mb, vram_used_mb, valid, optimizations):
+    """One test in the schema MetricsExporter::ExportToJSON writes (MetricsExporter.cpp:117-230): test_id as
+    TestConfiguration::GenerateTestId (:643-677), per-frame metrics, statistics and validation. Bandwidths are
+    the algorithmic bytes of the frame over its time, so device.bandwidth_estimated is set."""
+    ft = np.asarray(frame_ms, np.float64)
+    frames = [{"frame_num": i, "frame_time_ms": float(t), "fps": 1000.0 / float(t),
+               "bandwidth_read_gbps": read_bytes / (t * 1e-3) / 1e9,
+               "bandwidth_write_gbps": write_bytes / (t * 1e-3) / 1e9,
+               "ray_throughput_mrays": rays / (t * 1e-3) / 1e6, "vram_mb": float(vram_used_mb),
+               "avg_voxels_per_ray": float(iters_per_ray)}  # the reference stores ESVO iterations here
+              for i, t in enumerate(ft)]
+    shader = "vx_render"
+    return {
+        "test_id": f"hip_{voxel_res}_{scene_type.upper()}_{shader.upper()}_RUN1",
+        "timestamp": time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime()),
+        "configuration": {"pipeline": "hip", "shader": shader, "resolution": voxel_res, "scene_type": scene_type,
+                          "screen_width": width, "screen_height": height, "optimizations": list(optimizations)},
+        "device": {"gpu": gpu_name, "driver": driver, "vram_gb": vram_total_mb / 1024.0, "bandwidth_estimated": True},
+        "frames": frames,
+        "statistics": {"frame_time_mean": float(ft.mean()), "frame_time_stddev": float(ft.std()),
+                       "frame_time_p99": float(np.percentile(ft, 99)), "fps_mean": float(np.mean(1000.0 / ft)),
+                       "bandwidth_mean": float(np.mean([f["bandwidth_read_gbps"] for f in frames]))},
+        "validation": {"valid": bool(valid), "error_count": 0 if valid else 1, "warning_count": 0},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +301,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent-ray leg")
+    ap.add_argument("--vixen-json", default=None, help="also write the headline frames in VIXEN's results schema")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -355,6 +391,25 @@ def main() -> int:
         result["parity"] = "bit-exact vs oracle" if diff == 0 else f"{diff} records differ from the oracle"
     else:
         result["cpu_baseline"] = None
+    if args.vixen_json:
+        # per-frame times of the headline launch, each bracketed by its own events
+        scene = vx.Scene(ctx, built)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        for a, b in ev:
+            a.record(stream)
+            vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(cam), W, H, 1e30, hits.data_ptr(), None,
+                                       stream.cuda_stream), "vx_render")
+            b.record(stream)
+        torch.cuda.synchronize()
+        free, total = torch.cuda.mem_get_info(0)
+        rep_ = vixen_report([a.elapsed_time(b) for a, b in ev], rays, bytes_launch - 32 * rays, 32 * rays,
+                            c[1] / rays, N, "tunnels", W, H, torch.cuda.get_device_name(0),
+                            f"HIP {torch.version.hip}", total / 2 ** 20, (total - free) / 2 ** 20,
+                            result.get("parity", "bit-exact vs oracle") == "bit-exact vs oracle",
+                            ["esvo", "brick_dda", "bc1", "xcd_tile_remap", "lds_stack"])
+        scene.close()
+        Path(args.vixen_json).write_text(json.dumps(rep_, indent=2))
+        log(f"[bench] wrote {args.vixen_json}")
     print(json.dumps(result), flush=True)
     return 0
 
