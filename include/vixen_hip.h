@@ -85,6 +85,16 @@ typedef struct vx_ray {
  *   flags bits 4-5   : axis of the brick-DDA step into the hit voxel (1 x, 2 y,
  *                      3 z); 0 when the hit voxel is the first one the DDA tests
  *                      in its brick (the compressed shader's axisMask)
+ *   flags bit 6      : LOD hit (vx_trace_lod only): the traversal stopped at a
+ *                      coarse child whose projected pixel covers it; the record
+ *                      then has the reference's placeholder normal (0,1,0), the
+ *                      child's user level in bits 8-12, t = world distance of the
+ *                      child's entry, leaf_node = its parent descriptor, brick
+ *                      and voxel ~0u and rgb 0 (ISVOStructure::RayHit as built
+ *                      at libraries/SVO/src/SVOTraversal.cpp:637-647)
+ *   flags bits 8-12  : an LOD hit's user scale, esvoToUserScale(scale) =
+ *                      scale - (23 - user_max_levels): user_max_levels - 1 for
+ *                      the root's children, one less per level down
  *   flags bits 16-31 : ESVO loop iterations executed (hit or miss)
  *   t                : world-space distance to the entry of the hit voxel's AABB,
  *                      clamped to the start of the traced segment; +inf on a miss
@@ -102,6 +112,8 @@ typedef struct vx_hit {
 } vx_hit;
 
 #define VX_HIT_FLAG_HIT 1u
+#define VX_HIT_FLAG_LOD (1u << 6)
+#define VX_HIT_LOD_LEVEL(flags) (((flags) >> 8) & 31u)
 #define VX_HIT_AXIS(flags) (((flags) >> 1) & 3u)
 #define VX_HIT_ITERS(flags) ((flags) >> 16)
 
@@ -208,6 +220,33 @@ vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t heigh
  * missed primary gets a zero direction (a miss when traced). n device rays. */
 vx_status vx_bounce_rays(const vx_scene* scene, const vx_ray* primary, const vx_hit* hits, uint64_t n,
                          uint32_t seed, vx_ray* out, vx_stream stream);
+
+/* Screen-space LOD: a ray cone of diameter ray_orig_size at the origin that
+ * grows by ray_dir_size per unit distance (LODParameters,
+ * libraries/SVO/include/SVOLOD.h:50-177). LOD is enabled when ray_dir_size > 0. */
+typedef struct vx_lod {
+    float ray_orig_size;
+    float ray_dir_size;
+} vx_lod;
+
+/* LODParameters::fromCamera / fromCameraWithNearPlane / withBias
+ * (SVOLOD.h:83-90, 105-116, 169-175): dir = 2 tan(fov_y / screen_height / 2), orig = near * dir,
+ * a bias scales both by 2^bias. fov_y in radians. Host-only arithmetic. */
+vx_lod vx_lod_from_camera(float fov_y, int32_t screen_height);
+vx_lod vx_lod_from_camera_near(float fov_y, int32_t screen_height, float near_plane);
+vx_lod vx_lod_with_bias(vx_lod lod, float bias);
+
+/* Batch castRayWithLOD (libraries/SVO/src/SVOTraversal.cpp:224-232; with
+ * fromCamera parameters it is castRayScreenSpaceLOD, :212-222): vx_trace with
+ * the termination test of castRayImpl :614-650 before every child's leaf/PUSH
+ * decision: stop when (tstart + t_min |grid_max - grid_min|) * ray_dir_size +
+ * ray_orig_size >= the child's edge 2^(scale-23) (grid_max.x - grid_min.x),
+ * where tstart is the traced segment's start and t_min the child's entry in the
+ * unit cube. Such stops are LOD hits (VX_HIT_FLAG_LOD); other records are
+ * exactly vx_trace's. With LOD disabled (ray_dir_size <= 0 or NaN) this is
+ * vx_trace. lod is a host pointer. */
+vx_status vx_trace_lod(vx_scene* scene, const vx_ray* rays, uint64_t n, const vx_lod* lod, vx_hit* hits,
+                       vx_stream stream);
 
 /* Per-launch traversal counters for the algorithmic-bytes model:
  * counters[0] rays, [1] ESVO iterations (8-byte descriptor fetch each),

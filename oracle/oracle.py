@@ -40,6 +40,7 @@ def _load() -> C.CDLL:
     vp, u64 = C.c_void_p, C.c_uint64
     lib.orc_trace.argtypes = [C.POINTER(OrcScene), vp, u64, vp, vp]
     lib.orc_trace_mt.argtypes = [C.POINTER(OrcScene), vp, u64, vp, vp, C.c_int]
+    lib.orc_trace_lod.argtypes = [C.POINTER(OrcScene), vp, u64, C.c_float, C.c_float, vp, C.c_int]
     lib.orc_camera_rays.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, vp]
     lib.orc_bc1_decode.argtypes = [C.c_uint64, C.c_int, vp]
     lib.orc_rgba8_decode.argtypes = [C.c_uint32, vp]
@@ -51,7 +52,7 @@ def _load() -> C.CDLL:
     lib.orc_shade.argtypes = [C.POINTER(OrcScene), vp, vp, C.c_float]
     lib.orc_shade.restype = C.c_uint32
     lib.orc_render.argtypes = [C.POINTER(OrcScene), vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_int]
-    for f in ("orc_trace", "orc_trace_mt", "orc_camera_rays", "orc_bc1_decode", "orc_rgba8_decode",
+    for f in ("orc_trace", "orc_trace_mt", "orc_trace_lod", "orc_camera_rays", "orc_bc1_decode", "orc_rgba8_decode",
               "orc_bruteforce", "orc_bc1_decode_shader", "orc_dxt_normal_decode", "orc_render"):
         getattr(lib, f).restype = None
     return lib
@@ -105,6 +106,16 @@ def trace(scene: OracleScene, rays: np.ndarray, nthreads: int = 1):
     else:
         lib().orc_trace(C.byref(scene.s), rays.ctypes.data, len(rays), out.ctypes.data, counts.ctypes.data)
     return out, counts
+
+
+def trace_lod(scene: OracleScene, rays: np.ndarray, ray_orig_size: float, ray_dir_size: float,
+              nthreads: int = 1) -> np.ndarray:
+    """castRayWithLOD over a batch (oracle/vx_oracle.c orc_trace_lod)."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    out = np.zeros(len(rays), HIT_DTYPE)
+    lib().orc_trace_lod(C.byref(scene.s), rays.ctypes.data, len(rays), ray_orig_size, ray_dir_size,
+                        out.ctypes.data, max(1, nthreads))
+    return out
 
 
 def camera_rays(cam_floats: np.ndarray, w: int, h: int, tmax: float) -> np.ndarray:

@@ -22,6 +22,11 @@
  *                     encode -> decode round trips (tests/test_shading.py)
  *   orc_shade         pinned by a float64 evaluation of every pixel of a frame
  *                     (within one 8-bit step)
+ *   orc_trace_lod     pinned by closed forms on hand-built grids (stop level,
+ *                     iteration count and distance at the entry face; the
+ *                     grid-diagonal distance past it), monotonicity in the cone
+ *                     width, and equality with orc_trace where LOD is disabled
+ *                     or never fires (tests/test_lod.py)
  *   orc_bruteforce    textbook Amanatides-Woo in FP64; pinned by random-ray
  *                     agreement with an exhaustive slab test per voxel on tiny grids
  */
@@ -256,7 +261,8 @@ static int brick_dda(const orc_scene* s, uint32_t brick, const int bc[3], const 
     return 0;
 }
 
-static void trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, uint64_t c[5])
+/* lod: NULL, or {ray_orig_size, ray_dir_size} of an enabled LODParameters */
+static void trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, uint64_t c[5], const float* lod)
 {
     c[0] += 1;
     set_miss(h, 0);
@@ -345,6 +351,31 @@ static void trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, uint64_t
             cy = half * tc[1] + ty;
             cz = half * tc[2] + tz;
             process = t_min <= tv_max + T_EPS;
+        }
+        if (process && lod) {
+            /* screen-space LOD termination, castRayImpl SVOTraversal.cpp:614-650:
+             * worldDistance = tRayStart + t_min * length(worldSize) (:545-552),
+             * worldVoxelSize = esvoScaleToWorldSize(scale, worldSize.x)
+             * (SVOLOD.h:189-195), terminate when worldDistance * rayDirSize +
+             * rayOrigSize >= worldVoxelSize (SVOLOD.h:129-146) */
+            const float ex = s->grid_max[0] - s->grid_min[0];
+            const float ey = s->grid_max[1] - s->grid_min[1];
+            const float ez = s->grid_max[2] - s->grid_min[2];
+            const float world_len = sqrtf((ex * ex + ey * ey) + ez * ez);
+            const float dist = tstart + t_min * world_len;
+            const float size = ldexpf(1.0f, scale - ESVO_MAX_SCALE - 1) * ex;
+            if (dist * lod[1] + lod[0] >= size) {
+                /* the LOD RayHit of :637-647: normal (0,1,0), tMin = worldDistance,
+                 * scale = esvoToUserScale(scale) */
+                h->flags = 1u | (1u << 1) | (1u << 3) | (1u << 6) | ((uint32_t)(scale - min_scale) << 8) |
+                           ((uint32_t)(iter + 1) << 16);
+                h->t = dist;
+                h->leaf_node = parent;
+                h->brick = 0xFFFFFFFFu;
+                h->voxel = 0xFFFFFFFFu;
+                h->rgb[0] = h->rgb[1] = h->rgb[2] = 0.0f;
+                return;
+            }
         }
         if (process) {
             if ((leaf >> local) & 1u) {
@@ -447,7 +478,7 @@ static void trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, uint64_t
 void orc_trace(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out, uint64_t counts[5])
 {
     uint64_t c[5] = {0, 0, 0, 0, 0};
-    for (uint64_t i = 0; i < n; ++i) trace_one(s, &rays[i], &out[i], c);
+    for (uint64_t i = 0; i < n; ++i) trace_one(s, &rays[i], &out[i], c, NULL);
     if (counts)
         for (int k = 0; k < 5; ++k) counts[k] += c[k];
 }
@@ -458,18 +489,19 @@ typedef struct {
     orc_hit* out;
     uint64_t begin, end;
     uint64_t c[5];
+    const float* lod;
 } orc_job;
 
 static void* orc_worker(void* p)
 {
     orc_job* j = (orc_job*)p;
     memset(j->c, 0, sizeof j->c);
-    for (uint64_t i = j->begin; i < j->end; ++i) trace_one(j->s, &j->rays[i], &j->out[i], j->c);
+    for (uint64_t i = j->begin; i < j->end; ++i) trace_one(j->s, &j->rays[i], &j->out[i], j->c, j->lod);
     return NULL;
 }
 
-void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out, uint64_t counts[5],
-                  int nthreads)
+static void trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out, uint64_t counts[5],
+                     int nthreads, const float* lod)
 {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
@@ -480,6 +512,7 @@ void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* 
         jobs[t].s = s;
         jobs[t].rays = rays;
         jobs[t].out = out;
+        jobs[t].lod = lod;
         jobs[t].begin = per * (uint64_t)t < n ? per * (uint64_t)t : n;
         jobs[t].end = per * (uint64_t)(t + 1) < n ? per * (uint64_t)(t + 1) : n;
         pthread_create(&th[t], NULL, orc_worker, &jobs[t]);
@@ -489,6 +522,19 @@ void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* 
         if (counts)
             for (int k = 0; k < 5; ++k) counts[k] += jobs[t].c[k];
     }
+}
+
+void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out, uint64_t counts[5],
+                  int nthreads)
+{
+    trace_mt(s, rays, n, out, counts, nthreads, NULL);
+}
+
+void orc_trace_lod(const orc_scene* s, const orc_ray* rays, uint64_t n, float ray_orig_size, float ray_dir_size,
+                   orc_hit* out, int nthreads)
+{
+    const float lod[2] = {ray_orig_size, ray_dir_size};
+    trace_mt(s, rays, n, out, NULL, nthreads, ray_dir_size > 0.0f ? lod : NULL); /* LODParameters::isEnabled */
 }
 
 /* ---------------------------------------------------------------------- */

@@ -64,6 +64,12 @@ void orc_trace(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out
 void orc_trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hit* out, uint64_t counts[5],
                   int nthreads);
 
+/* castRayWithLOD (SVOTraversal.cpp:224-232) with LODParameters{ray_orig_size,
+ * ray_dir_size}; ray_dir_size <= 0 disables LOD (then it is orc_trace). An LOD
+ * stop is a hit with flags bit 6 set and the child's user level in bits 8-12. */
+void orc_trace_lod(const orc_scene* s, const orc_ray* rays, uint64_t n, float ray_orig_size, float ray_dir_size,
+                   orc_hit* out, int nthreads);
+
 /* The 64-byte push constant as 16 floats/ints: pos[3], time, dir[3], fov_deg,
  * up[3], aspect, right[3], debug_mode. Writes w*h rays, row-major. */
 void orc_camera_rays(const float cam[16], uint32_t w, uint32_t h, float tmax, orc_ray* out);

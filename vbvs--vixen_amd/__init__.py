@@ -108,6 +108,14 @@ class VxBuiltScene(C.Structure):
                 ("internal_count", C.c_uint64), ("config", VxOctreeConfig)]
 
 
+class VxLod(C.Structure):
+    """LODParameters (libraries/SVO/include/SVOLOD.h:50-177)."""
+    _fields_ = [("ray_orig_size", C.c_float), ("ray_dir_size", C.c_float)]
+
+    def enabled(self) -> bool:
+        return self.ray_dir_size > 0.0
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("flags", "<u4"), ("t", "<f4"), ("leaf_node", "<u4"), ("brick", "<u4"),
                       ("voxel", "<u4"), ("rgb", "<f4", 3)])
@@ -139,6 +147,10 @@ def _load() -> C.CDLL:
         "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
         "vx_trace": ([vp, vp, u64, vp, vp], st),
         "vx_trace_counters": ([vp, vp, u64, vp, vp, vp], st),
+        "vx_trace_lod": ([vp, vp, u64, C.POINTER(VxLod), vp, vp], st),
+        "vx_lod_from_camera": ([C.c_float, i32], VxLod),
+        "vx_lod_from_camera_near": ([C.c_float, i32, C.c_float], VxLod),
+        "vx_lod_with_bias": ([VxLod, C.c_float], VxLod),
         "vx_render": ([vp, C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp, vp], st),
         "vx_camera_rays": ([C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp], st),
         "vx_bounce_rays": ([vp, vp, vp, u64, u32, vp, vp], st),
@@ -185,7 +197,20 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
             "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
             "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
-            "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write")
+            "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
+            "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias")
+
+
+def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:
+    """LODParameters::fromCamera / fromCameraWithNearPlane (fov_y in radians)."""
+    if near_plane is None:
+        return LIB.vx_lod_from_camera(fov_y, screen_height)
+    return LIB.vx_lod_from_camera_near(fov_y, screen_height, near_plane)
+
+
+def lod_with_bias(lod: VxLod, bias: float) -> VxLod:
+    """LODParameters::withBias: both cone sizes times 2^bias."""
+    return LIB.vx_lod_with_bias(lod, bias)
 
 
 def _check(status: int, where: str) -> None:
@@ -501,6 +526,16 @@ class Scene:
         if hits is None:
             hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
         _check(LIB.vx_trace(self._h, rays.data_ptr(), n, hits.data_ptr(), _stream_ptr(stream)), "vx_trace")
+        return hits
+
+    def trace_lod(self, rays, lod: VxLod, hits=None, stream=None):
+        """castRayWithLOD over a batch; LOD hits carry VX_HIT_FLAG_LOD (bit 6)."""
+        torch = _torch()
+        n = rays.numel() * rays.element_size() // 32
+        if hits is None:
+            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        _check(LIB.vx_trace_lod(self._h, rays.data_ptr(), n, C.byref(lod), hits.data_ptr(), _stream_ptr(stream)),
+               "vx_trace_lod")
         return hits
 
     def trace_counters(self, rays, hits=None, stream=None):

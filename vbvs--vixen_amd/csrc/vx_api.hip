@@ -84,7 +84,7 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
     if ((threadIdx.x & 63u) == 0u) atomicAdd(dst, x);
 }
 
-template <bool COUNT, bool STAGED>
+template <bool COUNT, bool STAGED, bool LOD = false>
 __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __restrict__ rays, uint64_t n,
                                                   uint32_t per_xcd, vx_hit* __restrict__ hits,
                                                   unsigned long long* __restrict__ counters)
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
         float o[3], d[3], tmin, tmax;
         load_ray(rays, i, o, d, tmin, tmax);
         vx_hit h;
-        trace_ray<COUNT, STAGED, true>(s, o, d, tmin, tmax, h, cnt);
+        trace_ray<COUNT, STAGED, true, LOD>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
     if (COUNT) {
@@ -505,6 +505,16 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.invN = 1.0f / v.Nf;
     v.Bf = static_cast<float>(B);
     v.min_scale = 22 - c.user_max_levels + 1;
+    {
+        // castRayImpl's worldRayLength = glm::length(worldMax - worldMin) and
+        // worldSize.x (libraries/SVO/src/SVOTraversal.cpp:547-552)
+        const float ex = c.grid_max[0] - c.grid_min[0], ey = c.grid_max[1] - c.grid_min[1],
+                    ez = c.grid_max[2] - c.grid_min[2];
+        v.lod_ray_length = std::sqrt((ex * ex + ey * ey) + ez * ez);
+        v.lod_world_size = ex;
+        v.lod_orig = 0.0f;
+        v.lod_dir = 0.0f;
+    }
     v.lds_off = 0;
     v.lds_count = 0;
     v.stack_depth = static_cast<uint32_t>(std::max(1, ilog2(B) - 1));
@@ -528,7 +538,7 @@ static SceneView launch_view(const vx_scene* s, uint32_t& lds)
 }
 
 static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
-                              hipStream_t st)
+                              hipStream_t st, const vx_lod* lod = nullptr)
 {
     if (!s || (n && (!rays || !hits))) return fail(VX_ERR_INVALID_ARG, "vx_trace: null argument");
     if (n == 0) return VX_OK;
@@ -537,9 +547,14 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
     uint32_t lds = 0;
-    const SceneView v = launch_view(s, lds);
+    SceneView v = launch_view(s, lds);
     const bool staged = v.lds_count > 0;
-    if (counters) {
+    if (lod && lod->ray_dir_size > 0.0f) { // LODParameters::isEnabled
+        v.lod_orig = lod->ray_orig_size;
+        v.lod_dir = lod->ray_dir_size;
+        if (staged) k_trace<false, true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        else k_trace<false, false, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+    } else if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
         auto* c = reinterpret_cast<unsigned long long*>(counters);
         if (staged) k_trace<true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
@@ -641,6 +656,34 @@ vx_status vx_scene_set_lds_depth(vx_scene* s, uint32_t depth)
 vx_status vx_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, vx_stream stream)
 {
     return vx::launch_trace(s, rays, n, hits, nullptr, static_cast<hipStream_t>(stream));
+}
+
+vx_status vx_trace_lod(vx_scene* s, const vx_ray* rays, uint64_t n, const vx_lod* lod, vx_hit* hits,
+                       vx_stream stream)
+{
+    if (!lod) return vx::fail(VX_ERR_INVALID_ARG, "vx_trace_lod: null lod");
+    return vx::launch_trace(s, rays, n, hits, nullptr, static_cast<hipStream_t>(stream), lod);
+}
+
+// LODParameters::fromCamera (libraries/SVO/include/SVOLOD.h:83-90)
+vx_lod vx_lod_from_camera(float fov_y, int32_t screen_height)
+{
+    const float pixel_angle = fov_y / static_cast<float>(screen_height);
+    return vx_lod{0.0f, 2.0f * std::tan(pixel_angle * 0.5f)};
+}
+
+// LODParameters::fromCameraWithNearPlane (SVOLOD.h:105-116)
+vx_lod vx_lod_from_camera_near(float fov_y, int32_t screen_height, float near_plane)
+{
+    const float dir = vx_lod_from_camera(fov_y, screen_height).ray_dir_size;
+    return vx_lod{near_plane * dir, dir};
+}
+
+// LODParameters::withBias (SVOLOD.h:169-175)
+vx_lod vx_lod_with_bias(vx_lod lod, float bias)
+{
+    const float m = std::pow(2.0f, bias);
+    return vx_lod{lod.ray_orig_size * m, lod.ray_dir_size * m};
 }
 
 vx_status vx_trace_counters(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,

@@ -68,6 +68,9 @@ struct SceneView {
     uint32_t stack_base; // first LDS word of the per-lane traversal stacks
     uint32_t stack_depth; // stack entries per lane (levels above the bricks - 1, >= 1)
     uint32_t aux_base;   // first LDS word of the per-lane brick-DDA inputs
+    // screen-space LOD (LOD kernels only): ray cone size at the origin and
+    // growth per unit distance, |grid_max - grid_min|, and the grid's x extent
+    float lod_orig, lod_dir, lod_ray_length, lod_world_size;
 };
 
 struct TraceCounts {
@@ -544,7 +547,11 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
 
 // IMMEDIATE: visit a leaf's brick inside the iteration (a miss falls through
 // to ADVANCE, as in the reference); otherwise return STEP_LEAF with lf set.
-template <bool COUNT, bool STAGED, bool IMMEDIATE>
+// LOD: screen-space LOD termination before the leaf/PUSH decision
+// (libraries/SVO/src/SVOTraversal.cpp:614-650, LODParameters::shouldTerminate
+// libraries/SVO/include/SVOLOD.h:129-146,189-195): the child is accepted when the ray
+// cone at distance tstart + t_min * |grid extent| is at least its world size.
+template <bool COUNT, bool STAGED, bool IMMEDIATE, bool LOD = false>
 __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt, LeafRef& lf)
 {
     const int iter = r.iter;
@@ -575,6 +582,22 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit
     const float cy = half * r.tc[1] + ty;
     const float cz = half * r.tc[2] + tz;
     const bool process = ((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS) && r.t_min <= tv_max + T_EPS;
+    if (LOD && process) {
+        const float dist = u2f(*lane_word(s.aux_base, 9)) + r.t_min * s.lod_ray_length;
+        // the child's edge is scale_exp2 = 2^(scale - 23) of the unit cube (esvoScaleToWorldSize)
+        if (dist * s.lod_dir + s.lod_orig >= r.scale_exp2 * s.lod_world_size) {
+            // a coarse hit at this child: the reference's placeholder normal (0,1,0),
+            // its user level (esvoToUserScale) in bits 8-12, its parent descriptor
+            h.flags = 1u | (1u << 1) | (1u << 3) | VX_HIT_FLAG_LOD | ((uint32_t)(r.scale - s.min_scale) << 8) |
+                      ((uint32_t)(iter + 1) << 16);
+            h.t = dist;
+            h.leaf_node = r.parent;
+            h.brick = 0xFFFFFFFFu;
+            h.voxel = 0xFFFFFFFFu;
+            h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
+            return STEP_DONE;
+        }
+    }
     if (process) {
         if ((leaf >> local) & 1u) {
             if (COUNT) cnt.leaves += 1;
@@ -630,11 +653,11 @@ __device__ __forceinline__ bool resume_after_leaf(const SceneView& s, RayState& 
 
 // One whole reference iteration. Returns true when the ray is finished; h then
 // holds its record.
-template <bool COUNT, bool STAGED>
+template <bool COUNT, bool STAGED, bool LOD = false>
 __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
 {
     LeafRef lf;
-    return esvo_step<COUNT, STAGED, true>(s, r, h, cnt, lf) == STEP_DONE;
+    return esvo_step<COUNT, STAGED, true, LOD>(s, r, h, cnt, lf) == STEP_DONE;
 }
 
 // Full traversal of one ray per lane. Writes the hit record; returns true on a
@@ -645,14 +668,14 @@ __device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit
 // brick DDAs together, so the wave pays for the DDA loop once per batch of
 // leaves instead of once per lane's timing. Coherent camera rays reach leaves
 // together anyway; for them the immediate form (no bookkeeping) is faster.
-template <bool COUNT, bool STAGED, bool POSTPONE>
+template <bool COUNT, bool STAGED, bool POSTPONE, bool LOD = false>
 __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
                                           float tmax, vx_hit& h, TraceCounts& cnt)
 {
     RayState r;
     if (!POSTPONE) {
         if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
-        while (!step_ray<COUNT, STAGED>(s, r, h, cnt)) {
+        while (!step_ray<COUNT, STAGED, LOD>(s, r, h, cnt)) {
         }
         return (h.flags & 1u) != 0u;
     }
@@ -661,7 +684,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], 
     for (;;) {
         if (mode == ESVO) {
             LeafRef lf;
-            const int st = esvo_step<COUNT, STAGED, false>(s, r, h, cnt, lf);
+            const int st = esvo_step<COUNT, STAGED, false, LOD>(s, r, h, cnt, lf);
             mode = st == STEP_DONE ? DONE : (st == STEP_LEAF ? LEAF : ESVO);
             if (mode == LEAF) { // parked in the lane's LDS column while it waits
                 *lane_word(s.aux_base, 11) = lf.brick;
