@@ -62,6 +62,43 @@ __global__ void __launch_bounds__(kBlock) k_build_occ(const uint32_t* __restrict
     occ[i] = m;
 }
 
+// Per brick, the box of its occupied voxels from the 16 occupancy words (bit b
+// of word w is voxel 32w + b = x + 8y + 64z): lo x|y|z in bits 0-8, hi x|y|z in
+// bits 9-17. An empty brick gets the whole brick (the pretest then rejects
+// nothing and the DDA finds nothing).
+__global__ void __launch_bounds__(kBlock) k_build_box(const uint32_t* __restrict__ occ, uint64_t nb,
+                                                      uint32_t* __restrict__ box)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nb) return;
+    const uint4* m = reinterpret_cast<const uint4*>(occ + i * 16u);
+    uint32_t xs = 0u, ys = 0u, zs = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = m[q];
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = 4 * q + j; // z = w / 2, rows y = (w & 1) * 4 + byte
+            const uint32_t word = w4[j];
+            zs |= (word != 0u ? 1u : 0u) << (w >> 1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t row = (word >> (8 * k)) & 0xFFu;
+                xs |= row;
+                ys |= (row != 0u ? 1u : 0u) << ((w & 1) * 4 + k);
+            }
+        }
+    }
+    if (zs == 0u) {
+        box[i] = 7u << 9 | 7u << 12 | 7u << 15;
+        return;
+    }
+    const uint32_t lx = __builtin_ctz(xs), ly = __builtin_ctz(ys), lz = __builtin_ctz(zs);
+    const uint32_t hx = 31u - __builtin_clz(xs), hy = 31u - __builtin_clz(ys), hz = 31u - __builtin_clz(zs);
+    box[i] = lx | ly << 3 | lz << 6 | hx << 9 | hy << 12 | hz << 15;
+}
+
 __device__ __forceinline__ uint32_t xcd_chunk(uint32_t per_xcd)
 {
     return (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -466,6 +503,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
         (void)hipFree(s->nodes);
         (void)hipFree(s->brick_data);
         (void)hipFree(s->occ);
+        (void)hipFree(s->box);
         (void)hipFree(s->bc1);
         (void)hipFree(s->nrm);
         delete s;
@@ -476,7 +514,8 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
                  occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256,
                  nrm_bytes = std::max<uint64_t>(1, nb) * 512;
     if (hipMalloc(&s->nodes, node_bytes) != hipSuccess || hipMalloc(&s->brick_data, brick_bytes) != hipSuccess ||
-        hipMalloc(&s->occ, occ_bytes) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess) ||
+        hipMalloc(&s->occ, occ_bytes) != hipSuccess ||
+        hipMalloc(&s->box, std::max<uint64_t>(1, nb) * 4) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess) ||
         (d->dxt_normals && hipMalloc(&s->nrm, nrm_bytes) != hipSuccess))
         return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
     if (hipMemcpy(s->nodes, d->esvo_nodes, node_bytes, hipMemcpyHostToDevice) != hipSuccess ||
@@ -487,6 +526,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     if (nb) {
         const uint64_t words = nb * 16;
         k_build_occ<<<dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0>>>(s->brick_data, words, s->occ);
+        k_build_box<<<dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0>>>(s->occ, nb, s->box);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
             return cleanup(fail(VX_ERR_HIP, "vx_scene_create: occupancy mask build failed"));
     }
@@ -494,6 +534,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.nodes = s->nodes;
     v.brick_data = s->brick_data;
     v.occ = s->occ;
+    v.box = s->box;
     v.bc1 = d->bc1_colors ? s->bc1 : nullptr;
     v.nrm = d->dxt_normals ? s->nrm : nullptr;
     v.node_count = static_cast<uint32_t>(d->node_count);
@@ -629,6 +670,7 @@ vx_status vx_scene_destroy(vx_scene* s)
     (void)hipFree(s->nodes);
     (void)hipFree(s->brick_data);
     (void)hipFree(s->occ);
+    (void)hipFree(s->box);
     (void)hipFree(s->bc1);
     (void)hipFree(s->nrm);
     delete s;

@@ -150,6 +150,7 @@ struct vx_batch {
     uint2* nodes = nullptr;
     uint32_t* brick_data = nullptr;
     uint32_t* occ = nullptr;
+    uint32_t* box = nullptr;
     uint2* bc1 = nullptr;
     // launch state of vx_trace_batched
     vx::BatchSceneInfo* d_table = nullptr;
@@ -289,6 +290,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         (void)hipFree(b->nodes);
         (void)hipFree(b->brick_data);
         (void)hipFree(b->occ);
+        (void)hipFree(b->box);
         (void)hipFree(b->bc1);
         (void)hipFree(b->d_table);
         (void)hipFree(b->d_queue);
@@ -298,6 +300,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
     if (hipMalloc(&b->nodes, nodes * 8) != hipSuccess ||
         hipMalloc(&b->brick_data, std::max<uint64_t>(1, bricks) * 2048) != hipSuccess ||
         hipMalloc(&b->occ, std::max<uint64_t>(1, bricks) * 64) != hipSuccess ||
+        hipMalloc(&b->box, std::max<uint64_t>(1, bricks) * 4) != hipSuccess ||
         hipMalloc(&b->bc1, std::max<uint64_t>(1, bc1_bricks) * 256) != hipSuccess ||
         hipMalloc(&b->d_table, n * sizeof(BatchSceneInfo)) != hipSuccess ||
         hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess)
@@ -309,7 +312,8 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
             (sc->brick_count &&
              (hipMemcpy(b->brick_data + bb * 512, sc->brick_data, sc->brick_count * 2048, hipMemcpyDeviceToDevice) !=
                   hipSuccess ||
-              hipMemcpy(b->occ + bb * 16, sc->occ, sc->brick_count * 64, hipMemcpyDeviceToDevice) != hipSuccess)) ||
+              hipMemcpy(b->occ + bb * 16, sc->occ, sc->brick_count * 64, hipMemcpyDeviceToDevice) != hipSuccess ||
+              hipMemcpy(b->box + bb, sc->box, sc->brick_count * 4, hipMemcpyDeviceToDevice) != hipSuccess)) ||
             (sc->view.bc1 && sc->brick_count &&
              hipMemcpy(b->bc1 + cb * 32, sc->bc1, sc->brick_count * 256, hipMemcpyDeviceToDevice) != hipSuccess))
             return cleanup(fail(VX_ERR_HIP, "vx_batch_create: pool copy failed"));
@@ -317,6 +321,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         v.nodes = b->nodes + nb;
         v.brick_data = b->brick_data + bb * 512;
         v.occ = b->occ + bb * 16;
+        v.box = b->box + bb;
         v.bc1 = sc->view.bc1 ? b->bc1 + cb * 32 : nullptr;
         v.nrm = nullptr; // the batched launch writes hit records only (no shading)
         v.lds_off = 0;
@@ -442,6 +447,7 @@ vx_status vx_batch_destroy(vx_batch* b)
     (void)hipFree(b->nodes);
     (void)hipFree(b->brick_data);
     (void)hipFree(b->occ);
+    (void)hipFree(b->box);
     (void)hipFree(b->bc1);
     (void)hipFree(b->d_table);
     (void)hipFree(b->d_queue);

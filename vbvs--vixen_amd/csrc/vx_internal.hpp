@@ -24,6 +24,7 @@ struct vx_scene {
     uint2* nodes;
     uint32_t* brick_data;
     uint32_t* occ;
+    uint32_t* box; // per brick: box of its occupied voxels (see SceneView::box)
     uint2* bc1;
     uint4* nrm; // DXT normal blocks (32 per brick) or nullptr
     uint64_t node_count;
