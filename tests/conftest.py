@@ -68,3 +68,22 @@ def record_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     wa = a.view(np.uint32).reshape(-1, 8)
     wb = b.view(np.uint32).reshape(-1, 8)
     return np.nonzero((wa != wb).any(1))[0]
+
+
+def leaving_rays(ray_dtype, N, n_per, seed):
+    """Origins within 2^-24 N of a max face, heading out through it: the set-up
+    succeeds but the normalised start rounds onto the face, so the ESVO interval
+    is empty (a miss after 0 iterations, decided after the ray inputs are
+    written)."""
+    rng = np.random.default_rng(seed)
+    near = np.float32(N) * np.float32(1 - 2 ** -24)
+    rays = np.zeros(3 * n_per, ray_dtype)
+    for a in range(3):
+        sl = slice(a * n_per, (a + 1) * n_per)
+        o = rng.uniform(1, N - 1, (n_per, 3)).astype(np.float32)
+        o[:, a] = near
+        d = rng.normal(size=(n_per, 3))
+        d[:, a] = np.abs(d[:, a]) + 0.1
+        rays["o"][sl], rays["d"][sl] = o, d
+    rays["tmax"] = 3.0e38
+    return rays

@@ -6,6 +6,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from conftest import leaving_rays
+
 pytestmark = pytest.mark.gpu
 
 
@@ -40,6 +42,8 @@ def batch_setup(vx, orc, gpu_ctx):
         r["o"][n // 2:] = rng.uniform(-0.2 * N, 1.2 * N, (n - n // 2, 3))
         r["d"][n // 2:] = rng.normal(size=(n - n // 2, 3))
         r["tmax"][n // 2:] = 1e30
+        if kind in ("cave", "urban") and n > 2000:
+            r[-300:] = leaving_rays(orc.RAY_DTYPE, N, 100, 200 + i)  # empty-interval exits
         rays_all.append(r)
         counts.append(n)
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import global_voxels, record_diff
+from conftest import global_voxels, leaving_rays, record_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +116,22 @@ def test_trace_edge_rays_bit_exact(vx, orc, cave, gpu_ctx):
     want, _ = orc.trace(orc.OracleScene.from_built(built), rays)
     _assert_same(got, want, rays, "edge rays")
     assert not (got["flags"][[0, 1, 2, 7, 8, 13]] & 1).any()
+
+
+@pytest.mark.parametrize("scene_name", ["cave", "cornell"])
+def test_rays_leaving_through_a_max_face_bit_exact(vx, orc, scene_name, request, gpu_ctx):
+    """Regression: a ray finished by initTraversalState's empty-interval exit
+    must get a miss record (not its own set-up values read back as a hit)."""
+    _, _, built = request.getfixturevalue(scene_name)
+    rays = leaving_rays(orc.RAY_DTYPE, built.N, 2000, 17)
+    osc = orc.OracleScene.from_built(built)
+    want, _ = orc.trace(osc, rays, nthreads=8)
+    assert (want["flags"] == 0).mean() > 0.5  # most take the empty-interval exit
+    print(f"leaving rays {scene_name}", flush=True)
+    scene = vx.Scene(gpu_ctx, built)
+    got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+    _sync()
+    _assert_same(got, want, rays, "leaving rays")
 
 
 def test_counters_match_oracle(vx, orc, cave, gpu_ctx):

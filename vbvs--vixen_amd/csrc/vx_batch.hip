@@ -122,8 +122,12 @@ __global__ void __launch_bounds__(kMaxGroup)
                     ray_id = cur + rank;
                     float o[3], d[3], tmin, tmax;
                     load_ray(rays, ray_id, o, d, tmin, tmax);
-                    if (init_ray(sv, o, d, tmin, tmax, r, h)) store_hit(hits, ray_id, h);
-                    else active = true;
+                    if (init_ray(sv, o, d, tmin, tmax, r)) {
+                        rec_load(sv, h);
+                        store_hit(hits, ray_id, h);
+                    } else {
+                        active = true;
+                    }
                 }
                 cur += take;
                 idle = __ballot(!active);
@@ -133,7 +137,8 @@ __global__ void __launch_bounds__(kMaxGroup)
             if (drained && pending == kNoChunk && cur == end) break;
             continue;
         }
-        if (active && step_ray<false, STAGED>(sv, r, h, cnt)) {
+        if (active && step_ray<false, STAGED>(sv, r, cnt)) {
+            rec_load(sv, h);
             store_hit(hits, ray_id, h);
             active = false;
         }

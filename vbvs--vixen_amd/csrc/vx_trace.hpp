@@ -194,14 +194,15 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
     return true;
 }
 
-// Brick DDA in local voxel space. Returns true on a hit and fills the hit record.
+// Brick DDA in local voxel space. Returns true on a hit, with the record's
+// hit/axis/sign/step bits in flags, the entry distance in t and the voxel.
 // rinv[i] is the ray's brick reciprocal, 1/dl[i] (+-1e8 when |dl[i]| < 1e-8),
 // computed once per ray: the reference's per-brick 1/dl and 1/|dl| are the
 // same correctly rounded values (RN(1/|x|) = |RN(1/x)|).
 template <bool COUNT>
 __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, const int bc[3], const float ol[3],
                                           const float dl[3], const float rinv[3], float tstart, float texit,
-                                          vx_hit& h, TraceCounts& cnt)
+                                          uint32_t& flags, float& t, uint32_t& voxel, TraceCounts& cnt)
 {
     float bmin[3], tn[3], tf[3];
 #pragma unroll
@@ -325,12 +326,9 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         axis = 2u;
         signpos = sz;
     }
-    h.flags = 1u | (axis << 1) | (signpos << 3) | (last << 4);
-    h.t = tHit;
-    h.brick = brick;
-    h.voxel = lin;
-    if (s.bc1) bc1_decode(s.bc1[(size_t)brick * 32u + (lin >> 4)], lin & 15u, h.rgb);
-    else rgba8_decode(s.brick_data[(size_t)brick * 512u + lin], h.rgb);
+    flags = 1u | (axis << 1) | (signpos << 3) | (last << 4);
+    t = tHit;
+    voxel = lin;
     return true;
 }
 
@@ -367,6 +365,42 @@ __device__ __forceinline__ void stack_put(const SceneView& s, int k, uint32_t pt
 __device__ __forceinline__ float stack_tmax(const SceneView& s, int k) { return u2f(*stack_word(s, k, true)); }
 __device__ __forceinline__ uint32_t stack_ptr(const SceneView& s, int k) { return *stack_word(s, k, false); }
 
+// A finished ray's record is written to words 0-4 of its lane's LDS column
+// (over the ray inputs, dead by then) and assembled into a vx_hit once the
+// traversal loop is left, so the loop carries no hit-record registers:
+//   word 0 flags (a miss: iterations << 16 only), 1 t, 2 leaf descriptor,
+//   3 brick, 4 voxel.
+__device__ __forceinline__ void rec_miss(const SceneView& s, uint32_t iters)
+{
+    *lane_word(s.aux_base, 0) = iters << 16;
+}
+__device__ __forceinline__ void rec_hit(const SceneView& s, uint32_t flags, float t, uint32_t leaf, uint32_t brick,
+                                        uint32_t voxel)
+{
+    *lane_word(s.aux_base, 0) = flags;
+    *lane_word(s.aux_base, 1) = f2u(t);
+    *lane_word(s.aux_base, 2) = leaf;
+    *lane_word(s.aux_base, 3) = brick;
+    *lane_word(s.aux_base, 4) = voxel;
+}
+// The record of the ray this lane finished, with the hit voxel's colour decoded.
+__device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
+{
+    const uint32_t f = *lane_word(s.aux_base, 0);
+    if (!(f & 1u)) {
+        store_miss(h, f >> 16);
+        return;
+    }
+    h.flags = f;
+    h.t = u2f(*lane_word(s.aux_base, 1));
+    h.leaf_node = *lane_word(s.aux_base, 2);
+    h.brick = *lane_word(s.aux_base, 3);
+    h.voxel = *lane_word(s.aux_base, 4);
+    if (f & VX_HIT_FLAG_LOD) h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
+    else if (s.bc1) bc1_decode(s.bc1[(size_t)h.brick * 32u + (h.voxel >> 4)], h.voxel & 15u, h.rgb);
+    else rgba8_decode(s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
+}
+
 // Per-lane LDS words beside the stack: the brick-DDA inputs ol[3], dl[3],
 // rinv[3], tstart, texit, then a pending leaf's brick id, brick cell (10 bits
 // per axis) and leaf descriptor index.
@@ -394,11 +428,11 @@ struct RayState {
 };
 
 // Ray set-up, coefficients and initTraversalState. Returns true when the ray
-// is already finished (a miss written to h); false when the loop must run.
+// is already finished (its miss record written); false when the loop must run.
 __device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], const float d[3], float tmin,
-                                         float tmax, RayState& r, vx_hit& h)
+                                         float tmax, RayState& r)
 {
-    store_miss(h, 0u);
+    rec_miss(s, 0u); // overwritten by the ray inputs when the set-up succeeds
     float ol[3], dl[3], du[3], invd[3], tstart, texit;
     if (!setup_ray(s, o, d, tmin, tmax, ol, dl, du, tstart, texit, invd)) return true;
     // brick-DDA inputs to this lane's LDS column (read back at brick visits)
@@ -433,7 +467,10 @@ __device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], c
     float t_min = mx(mx(2.0f * r.tc[0] - r.tb[0], 2.0f * r.tc[1] - r.tb[1]), 2.0f * r.tc[2] - r.tb[2]);
     const float t_max = mn(mn(r.tc[0] - r.tb[0], r.tc[1] - r.tb[1]), r.tc[2] - r.tb[2]);
     t_min = mx(t_min, 0.0f);
-    if (t_min >= t_max) return true;
+    if (t_min >= t_max) { // the ray inputs are in the record words now: write the miss over them
+        rec_miss(s, 0u);
+        return true;
+    }
 
     r.parent = 0u;
     r.scale = ESVO_MAX_SCALE;
@@ -493,9 +530,8 @@ struct LeafRef {
 
 // ADVANCE and POP of the iteration that started at r.iter, given the current
 // child's exit times tx, ty, tz and corrected max tcm. Returns true when the
-// ray is finished (a miss is written to h).
-__device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, vx_hit& h, float tx, float ty, float tz,
-                                            float tcm)
+// ray is finished (its miss record written).
+__device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, float tx, float ty, float tz, float tcm)
 {
     const int iter = r.iter;
     const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
@@ -518,7 +554,7 @@ __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, vx_
     if (r.scale >= ESVO_MAX_SCALE) {
         if (r.t_min > r.t_max || r.pos[0] < 1.0f || r.pos[0] >= 2.0f || r.pos[1] < 1.0f || r.pos[1] >= 2.0f ||
             r.pos[2] < 1.0f || r.pos[2] >= 2.0f) {
-            store_miss(h, (uint32_t)(iter + 1));
+            rec_miss(s, (uint32_t)(iter + 1));
             return true;
         }
         return false;
@@ -528,14 +564,14 @@ __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, vx_
     if (step_mask & 2) diff |= f2u(r.pos[1]) ^ f2u(r.pos[1] + r.scale_exp2);
     if (step_mask & 4) diff |= f2u(r.pos[2]) ^ f2u(r.pos[2] + r.scale_exp2);
     if (diff == 0u) {
-        store_miss(h, (uint32_t)(iter + 1));
+        rec_miss(s, (uint32_t)(iter + 1));
         return true;
     }
     const int scale = (int)((f2u((float)diff) >> 23) - 127u);
     r.scale = scale;
     r.scale_exp2 = u2f((uint32_t)(scale - ESVO_MAX_SCALE - 1 + 127) << 23);
     if (scale < s.min_scale || scale > ESVO_MAX_SCALE) {
-        store_miss(h, (uint32_t)(iter + 1));
+        rec_miss(s, (uint32_t)(iter + 1));
         return true;
     }
     r.parent = stack_ptr(s, ESVO_MAX_SCALE - scale);
@@ -548,10 +584,10 @@ __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, vx_
     return false;
 }
 
-// Brick DDA of a leaf child. True on a hit (h filled).
+// Brick DDA of a leaf child. True on a hit (its record written).
 template <bool COUNT>
 __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r, uint32_t brick, const int bc[3],
-                                           uint32_t lidx, vx_hit& h, TraceCounts& cnt)
+                                           uint32_t lidx, TraceCounts& cnt)
 {
     float ol[3], dl[3], rinv[3];
 #pragma unroll
@@ -561,9 +597,10 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
         rinv[i] = u2f(*lane_word(s.aux_base, 6 + i));
     }
     const float tstart = u2f(*lane_word(s.aux_base, 9)), texit = u2f(*lane_word(s.aux_base, 10));
-    if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, h, cnt)) return false;
-    h.flags |= (uint32_t)(r.iter + 1) << 16;
-    h.leaf_node = lidx;
+    uint32_t flags, voxel;
+    float t;
+    if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, flags, t, voxel, cnt)) return false;
+    rec_hit(s, flags | ((uint32_t)(r.iter + 1) << 16), t, lidx, brick, voxel);
     if (COUNT) cnt.hits += 1;
     return true;
 }
@@ -575,16 +612,16 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
 // libraries/SVO/include/SVOLOD.h:129-146,189-195): the child is accepted when the ray
 // cone at distance tstart + t_min * |grid extent| is at least its world size.
 template <bool COUNT, bool STAGED, bool IMMEDIATE, bool LOD = false>
-__device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt, LeafRef& lf)
+__device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceCounts& cnt, LeafRef& lf)
 {
     const int iter = r.iter;
     if (!(iter < MAX_ITERS && r.scale <= ESVO_MAX_SCALE)) {
-        store_miss(h, (uint32_t)iter);
+        rec_miss(s, (uint32_t)iter);
         return STEP_DONE;
     }
     if (COUNT) cnt.iters += 1;
     if (r.parent >= s.node_count) {
-        store_miss(h, (uint32_t)(iter + 1));
+        rec_miss(s, (uint32_t)(iter + 1));
         return STEP_DONE;
     }
     const uint2 desc = fetch_node<STAGED>(s, r.parent);
@@ -611,13 +648,10 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit
         if (dist * s.lod_dir + s.lod_orig >= r.scale_exp2 * s.lod_world_size) {
             // a coarse hit at this child: the reference's placeholder normal (0,1,0),
             // its user level (esvoToUserScale) in bits 8-12, its parent descriptor
-            h.flags = 1u | (1u << 1) | (1u << 3) | VX_HIT_FLAG_LOD | ((uint32_t)(r.scale - s.min_scale) << 8) |
-                      ((uint32_t)(iter + 1) << 16);
-            h.t = dist;
-            h.leaf_node = r.parent;
-            h.brick = 0xFFFFFFFFu;
-            h.voxel = 0xFFFFFFFFu;
-            h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
+            rec_hit(s,
+                    1u | (1u << 1) | (1u << 3) | VX_HIT_FLAG_LOD | ((uint32_t)(r.scale - s.min_scale) << 8) |
+                        ((uint32_t)(iter + 1) << 16),
+                    dist, r.parent, 0xFFFFFFFFu, 0xFFFFFFFFu);
             return STEP_DONE;
         }
     }
@@ -636,7 +670,7 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit
                         bc[i] = (int)((lp - 1.0f) * s.Bf);
                     }
                     if (IMMEDIATE) {
-                        if (visit_leaf<COUNT>(s, r, brick, bc, lidx, h, cnt)) return STEP_DONE;
+                        if (visit_leaf<COUNT>(s, r, brick, bc, lidx, cnt)) return STEP_DONE;
                     } else {
                         lf = LeafRef{brick, (bc[0] & 1023u) | ((bc[1] & 1023u) << 10) | ((bc[2] & 1023u) << 20), lidx};
                         return STEP_LEAF;
@@ -661,26 +695,26 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, vx_hit
             return STEP_CONTINUE;
         }
     }
-    return advance_ray(s, r, h, tx, ty, tz, tcm) ? STEP_DONE : STEP_CONTINUE;
+    return advance_ray(s, r, tx, ty, tz, tcm) ? STEP_DONE : STEP_CONTINUE;
 }
 
 // The rest of the iteration after a brick miss: ADVANCE from the leaf child
 // (its exit times are recomputed from the unchanged state).
-__device__ __forceinline__ bool resume_after_leaf(const SceneView& s, RayState& r, vx_hit& h)
+__device__ __forceinline__ bool resume_after_leaf(const SceneView& s, RayState& r)
 {
     const float tx = r.pos[0] * r.tc[0] - r.tb[0];
     const float ty = r.pos[1] * r.tc[1] - r.tb[1];
     const float tz = r.pos[2] * r.tc[2] - r.tb[2];
-    return advance_ray(s, r, h, tx, ty, tz, corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max));
+    return advance_ray(s, r, tx, ty, tz, corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max));
 }
 
-// One whole reference iteration. Returns true when the ray is finished; h then
-// holds its record.
+// One whole reference iteration. Returns true when the ray is finished; rec_load
+// then gives its record.
 template <bool COUNT, bool STAGED, bool LOD = false>
-__device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, vx_hit& h, TraceCounts& cnt)
+__device__ __forceinline__ bool step_ray(const SceneView& s, RayState& r, TraceCounts& cnt)
 {
     LeafRef lf;
-    return esvo_step<COUNT, STAGED, true, LOD>(s, r, h, cnt, lf) == STEP_DONE;
+    return esvo_step<COUNT, STAGED, true, LOD>(s, r, cnt, lf) == STEP_DONE;
 }
 
 // Full traversal of one ray per lane. Writes the hit record; returns true on a
@@ -697,17 +731,18 @@ __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], 
 {
     RayState r;
     if (!POSTPONE) {
-        if (init_ray(s, o, d, tmin, tmax, r, h)) return false;
-        while (!step_ray<COUNT, STAGED, LOD>(s, r, h, cnt)) {
-        }
+        if (!init_ray(s, o, d, tmin, tmax, r))
+            while (!step_ray<COUNT, STAGED, LOD>(s, r, cnt)) {
+            }
+        rec_load(s, h);
         return (h.flags & 1u) != 0u;
     }
     enum : int { ESVO = 0, LEAF = 1, DONE = 2 };
-    int mode = init_ray(s, o, d, tmin, tmax, r, h) ? DONE : ESVO;
+    int mode = init_ray(s, o, d, tmin, tmax, r) ? DONE : ESVO;
     for (;;) {
         if (mode == ESVO) {
             LeafRef lf;
-            const int st = esvo_step<COUNT, STAGED, false, LOD>(s, r, h, cnt, lf);
+            const int st = esvo_step<COUNT, STAGED, false, LOD>(s, r, cnt, lf);
             mode = st == STEP_DONE ? DONE : (st == STEP_LEAF ? LEAF : ESVO);
             if (mode == LEAF) { // parked in the lane's LDS column while it waits
                 *lane_word(s.aux_base, 11) = lf.brick;
@@ -722,11 +757,12 @@ __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], 
         if (waiting * 4u >= (uint32_t)__popcll(live) * 3u && mode == LEAF) {
             const uint32_t c = *lane_word(s.aux_base, 12);
             const int bc[3] = {(int)(c << 22) >> 22, (int)(c << 12) >> 22, (int)(c << 2) >> 22}; // 10-bit signed
-            if (visit_leaf<COUNT>(s, r, *lane_word(s.aux_base, 11), bc, *lane_word(s.aux_base, 13), h, cnt))
+            if (visit_leaf<COUNT>(s, r, *lane_word(s.aux_base, 11), bc, *lane_word(s.aux_base, 13), cnt))
                 mode = DONE;
-            else mode = resume_after_leaf(s, r, h) ? DONE : ESVO;
+            else mode = resume_after_leaf(s, r) ? DONE : ESVO;
         }
     }
+    rec_load(s, h);
     return (h.flags & 1u) != 0u;
 }
 
