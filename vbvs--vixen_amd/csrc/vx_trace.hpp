@@ -68,9 +68,10 @@ struct SceneView {
     int min_scale;
     uint32_t lds_off;    // first LDS slot of this scene's staged descriptors
     uint32_t lds_count;  // descriptors [0, lds_count) are read from LDS
-    uint32_t stack_base; // first LDS word of the per-lane traversal stacks
+    uint32_t stack_base; // first LDS word of the per-lane columns (stack, then aux words)
     uint32_t stack_depth; // stack entries per lane (levels above the bricks - 1, >= 1)
-    uint32_t aux_base;   // first LDS word of the per-lane brick-DDA inputs
+    uint32_t aux_base;   // offset of the brick-DDA inputs in a lane's column
+    uint32_t col_words;  // words per lane column (odd)
     // screen-space LOD (LOD kernels only): ray cone size at the origin and
     // growth per unit distance, |grid_max - grid_min|, and the grid's x extent
     float lod_orig, lod_dir, lod_ray_length, lod_world_size;
@@ -82,6 +83,10 @@ struct TraceCounts {
 
 __device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
+// One-instruction IEEE min/max. They equal mn/mx whenever the operands are not
+// NaN and not a (-0, +0) pair; every use states why that holds there.
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
@@ -234,8 +239,9 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
             const float hi = bmin[i] + ((float)((b >> (9 + 3 * i)) & 7u) + 1.25f);
             const float t0 = (lo - ol[i]) * rinv[i];
             const float t1 = (hi - ol[i]) * rinv[i];
-            en = mx(en, mn(t0, t1));
-            ex = mn(ex, mx(t0, t1));
+            // finite operands; the result only feeds en > ex, where -0 and +0 agree
+            en = fmx(en, fmn(t0, t1));
+            ex = fmn(ex, fmx(t0, t1));
         }
         if (en > ex) return false;
     }
@@ -334,10 +340,12 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
 
 // Descriptor fetch: a prefix of the descriptor array (the top levels of a
 // breadth-first tree) may be staged in LDS by the launching kernel.
-// The launching kernel's dynamic LDS: staged descriptors first, then per lane
-// the traversal stack (stack_depth parent pointers, then stack_depth t_max
-// words) and the kAuxWords brick-DDA inputs; word j of a lane's column is at
-// base + j * blockDim.x + threadIdx.x.
+// The launching kernel's dynamic LDS: staged descriptors first, then one column
+// of col_words consecutive words per lane: the traversal stack (stack_depth
+// parent pointers, then stack_depth t_max words) and the kAuxWords brick-DDA
+// inputs. Columns are an odd number of words apart, so the lanes of a wave
+// reading word j of their columns hit distinct banks, and every word of a
+// column is one base address plus an immediate offset.
 extern __shared__ uint2 vx_lds_nodes[];
 
 // STAGED is a compile-time choice: without staging the fetch is a plain global
@@ -349,13 +357,13 @@ __device__ __forceinline__ uint2 fetch_node(const SceneView& s, uint32_t i)
     return s.nodes[i];
 }
 
-__device__ __forceinline__ uint32_t* lane_word(uint32_t base, uint32_t j)
+__device__ __forceinline__ uint32_t* lane_word(const SceneView& s, uint32_t base, uint32_t j)
 {
-    return reinterpret_cast<uint32_t*>(vx_lds_nodes) + base + j * blockDim.x + threadIdx.x;
+    return reinterpret_cast<uint32_t*>(vx_lds_nodes) + s.stack_base + threadIdx.x * s.col_words + base + j;
 }
 __device__ __forceinline__ uint32_t* stack_word(const SceneView& s, int k, bool tmax)
 {
-    return lane_word(s.stack_base, (tmax ? s.stack_depth : 0u) + (uint32_t)k);
+    return lane_word(s, 0u, (tmax ? s.stack_depth : 0u) + (uint32_t)k);
 }
 __device__ __forceinline__ void stack_put(const SceneView& s, int k, uint32_t ptr, float t)
 {
@@ -372,46 +380,50 @@ __device__ __forceinline__ uint32_t stack_ptr(const SceneView& s, int k) { retur
 //   3 brick, 4 voxel.
 __device__ __forceinline__ void rec_miss(const SceneView& s, uint32_t iters)
 {
-    *lane_word(s.aux_base, 0) = iters << 16;
+    *lane_word(s, s.aux_base, 0) = iters << 16;
 }
 __device__ __forceinline__ void rec_hit(const SceneView& s, uint32_t flags, float t, uint32_t leaf, uint32_t brick,
                                         uint32_t voxel)
 {
-    *lane_word(s.aux_base, 0) = flags;
-    *lane_word(s.aux_base, 1) = f2u(t);
-    *lane_word(s.aux_base, 2) = leaf;
-    *lane_word(s.aux_base, 3) = brick;
-    *lane_word(s.aux_base, 4) = voxel;
+    *lane_word(s, s.aux_base, 0) = flags;
+    *lane_word(s, s.aux_base, 1) = f2u(t);
+    *lane_word(s, s.aux_base, 2) = leaf;
+    *lane_word(s, s.aux_base, 3) = brick;
+    *lane_word(s, s.aux_base, 4) = voxel;
 }
 // The record of the ray this lane finished, with the hit voxel's colour decoded.
 __device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
 {
-    const uint32_t f = *lane_word(s.aux_base, 0);
+    const uint32_t f = *lane_word(s, s.aux_base, 0);
     if (!(f & 1u)) {
         store_miss(h, f >> 16);
         return;
     }
     h.flags = f;
-    h.t = u2f(*lane_word(s.aux_base, 1));
-    h.leaf_node = *lane_word(s.aux_base, 2);
-    h.brick = *lane_word(s.aux_base, 3);
-    h.voxel = *lane_word(s.aux_base, 4);
+    h.t = u2f(*lane_word(s, s.aux_base, 1));
+    h.leaf_node = *lane_word(s, s.aux_base, 2);
+    h.brick = *lane_word(s, s.aux_base, 3);
+    h.voxel = *lane_word(s, s.aux_base, 4);
     if (f & VX_HIT_FLAG_LOD) h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
     else if (s.bc1) bc1_decode(s.bc1[(size_t)h.brick * 32u + (h.voxel >> 4)], h.voxel & 15u, h.rgb);
     else rgba8_decode(s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
 }
 
-// Per-lane LDS words beside the stack: the brick-DDA inputs ol[3], dl[3],
-// rinv[3], tstart, texit, then a pending leaf's brick id, brick cell (10 bits
-// per axis) and leaf descriptor index.
-constexpr uint32_t kAuxWords = 14;
-// LDS bytes per lane: stack (2 words per entry) and brick-DDA inputs.
-constexpr uint32_t lane_lds_bytes(uint32_t stack_depth) { return (2u * stack_depth + kAuxWords) * 4u; }
+// Per-lane LDS words after the stack: the brick-DDA inputs ol[3], dl[3],
+// rinv[3], tstart, texit, then a pending leaf's brick id and leaf descriptor
+// index (its brick cell is recomputed from the unchanged ray state).
+constexpr uint32_t kAuxWords = 13;
+// Words per lane column: stack (2 words per entry) and brick-DDA inputs; odd
+// for every depth since kAuxWords is odd.
+constexpr uint32_t lane_words(uint32_t stack_depth) { return 2u * stack_depth + kAuxWords; }
+static_assert(kAuxWords % 2u == 1u, "lane columns must be an odd number of words apart");
+constexpr uint32_t lane_lds_bytes(uint32_t stack_depth) { return lane_words(stack_depth) * 4u; }
 // Lays out the per-lane columns after `staged` descriptors; returns the kernel's LDS bytes.
 inline uint32_t lay_out_lanes(SceneView& v, uint32_t staged, uint32_t lanes)
 {
     v.stack_base = staged * 2u;
-    v.aux_base = v.stack_base + 2u * v.stack_depth * lanes;
+    v.aux_base = 2u * v.stack_depth;
+    v.col_words = lane_words(v.stack_depth);
     return staged * 8u + lanes * lane_lds_bytes(v.stack_depth);
 }
 
@@ -438,12 +450,12 @@ __device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], c
     // brick-DDA inputs to this lane's LDS column (read back at brick visits)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        *lane_word(s.aux_base, i) = f2u(ol[i]);
-        *lane_word(s.aux_base, 3 + i) = f2u(dl[i]);
-        *lane_word(s.aux_base, 6 + i) = f2u(__builtin_fabsf(dl[i]) < 1e-8f ? (dl[i] >= 0.0f ? 1e8f : -1e8f) : invd[i]);
+        *lane_word(s, s.aux_base, i) = f2u(ol[i]);
+        *lane_word(s, s.aux_base, 3 + i) = f2u(dl[i]);
+        *lane_word(s, s.aux_base, 6 + i) = f2u(__builtin_fabsf(dl[i]) < 1e-8f ? (dl[i] >= 0.0f ? 1e8f : -1e8f) : invd[i]);
     }
-    *lane_word(s.aux_base, 9) = f2u(tstart);
-    *lane_word(s.aux_base, 10) = f2u(texit);
+    *lane_word(s, s.aux_base, 9) = f2u(tstart);
+    *lane_word(s, s.aux_base, 10) = f2u(texit);
 
     float n0[3];
 #pragma unroll
@@ -508,7 +520,9 @@ __device__ __forceinline__ float corrected_tc_max_f(float tx, float ty, float tz
     const float vx = ((mask & 8) && __builtin_fabsf(tx) < thr) ? tx : t_max;
     const float vy = ((mask & 16) && __builtin_fabsf(ty) < thr) ? ty : t_max;
     const float vz = ((mask & 32) && __builtin_fabsf(tz) < thr) ? tz : t_max;
-    return mn(mn(vx, vy), vz);
+    // tx = pos*tc - tb with pos*tc <= -1 is finite and never -0; t_max is built
+    // from such differences (initTraversalState) and their minima
+    return fmn(fmn(vx, vy), vz);
 }
 
 // One iteration of the ESVO loop (VoxelRayMarch_Compressed.comp:396-478) is
@@ -522,11 +536,22 @@ __device__ __forceinline__ float corrected_tc_max_f(float tx, float ty, float tz
 // Chained without a pause (step_ray) they are exactly the reference iteration.
 enum : int { STEP_CONTINUE = 0, STEP_DONE = 1, STEP_LEAF = 2 };
 
-// A leaf child about to be visited: brick id, brick cell (10 signed bits per
-// axis) and the leaf descriptor's index.
+// A leaf child about to be visited: brick id and the leaf descriptor's index.
 struct LeafRef {
-    uint32_t brick, cell, lidx;
+    uint32_t brick, lidx;
 };
+
+// Grid cell of the brick under the current child (SVOBrickDDA.cpp:85-93): the
+// child's unmirrored corner in the unit cube times the bricks per axis.
+__device__ __forceinline__ void brick_cell(const SceneView& s, const RayState& r, int bc[3])
+{
+    const int octm = r.mask & 7;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
+        bc[i] = (int)((lp - 1.0f) * s.Bf);
+    }
+}
 
 // ADVANCE and POP of the iteration that started at r.iter, given the current
 // child's exit times tx, ty, tz and corrected max tcm. Returns true when the
@@ -536,12 +561,12 @@ __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, flo
     const int iter = r.iter;
     const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
     const float tca =
-        tcm >= 1e10f ? mx(mx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f) : tcm;
+        tcm >= 1e10f ? fmx(fmx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f) : tcm;
     int step_mask = 0;
     if (canx && tx <= tca) { step_mask ^= 1; r.pos[0] -= r.scale_exp2; }
     if (cany && ty <= tca) { step_mask ^= 2; r.pos[1] -= r.scale_exp2; }
     if (canz && tz <= tca) { step_mask ^= 4; r.pos[2] -= r.scale_exp2; }
-    r.t_min = mx(tca, 0.0f);
+    r.t_min = fmx(tca, 0.0f); // tca: finite, never -0 (see corrected_tc_max_f)
     r.idx ^= step_mask;
     r.iter = iter + 1;
 
@@ -592,11 +617,11 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
     float ol[3], dl[3], rinv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        ol[i] = u2f(*lane_word(s.aux_base, i));
-        dl[i] = u2f(*lane_word(s.aux_base, 3 + i));
-        rinv[i] = u2f(*lane_word(s.aux_base, 6 + i));
+        ol[i] = u2f(*lane_word(s, s.aux_base, i));
+        dl[i] = u2f(*lane_word(s, s.aux_base, 3 + i));
+        rinv[i] = u2f(*lane_word(s, s.aux_base, 6 + i));
     }
-    const float tstart = u2f(*lane_word(s.aux_base, 9)), texit = u2f(*lane_word(s.aux_base, 10));
+    const float tstart = u2f(*lane_word(s, s.aux_base, 9)), texit = u2f(*lane_word(s, s.aux_base, 10));
     uint32_t flags, voxel;
     float t;
     if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, flags, t, voxel, cnt)) return false;
@@ -636,14 +661,14 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
     const float ty = r.pos[1] * r.tc[1] - r.tb[1];
     const float tz = r.pos[2] * r.tc[2] - r.tb[2];
     const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
-    const float tv_max = mn(r.t_max, tcm);
+    const float tv_max = fmn(r.t_max, tcm); // finite, never -0 (see corrected_tc_max_f)
     const float half = r.scale_exp2 * 0.5f;
     const float cx = half * r.tc[0] + tx;
     const float cy = half * r.tc[1] + ty;
     const float cz = half * r.tc[2] + tz;
     const bool process = ((valid >> local) & 1u) && !(r.t_min > r.t_max + T_EPS) && r.t_min <= tv_max + T_EPS;
     if (LOD && process) {
-        const float dist = u2f(*lane_word(s.aux_base, 9)) + r.t_min * s.lod_ray_length;
+        const float dist = u2f(*lane_word(s, s.aux_base, 9)) + r.t_min * s.lod_ray_length;
         // the child's edge is scale_exp2 = 2^(scale - 23) of the unit cube (esvoScaleToWorldSize)
         if (dist * s.lod_dir + s.lod_orig >= r.scale_exp2 * s.lod_world_size) {
             // a coarse hit at this child: the reference's placeholder normal (0,1,0),
@@ -663,16 +688,12 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
             if (lidx < s.node_count) {
                 const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
                 if (brick != 0xFFFFFFu && brick < s.brick_count) {
-                    int bc[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
-                        bc[i] = (int)((lp - 1.0f) * s.Bf);
-                    }
                     if (IMMEDIATE) {
+                        int bc[3];
+                        brick_cell(s, r, bc);
                         if (visit_leaf<COUNT>(s, r, brick, bc, lidx, cnt)) return STEP_DONE;
                     } else {
-                        lf = LeafRef{brick, (bc[0] & 1023u) | ((bc[1] & 1023u) << 10) | ((bc[2] & 1023u) << 20), lidx};
+                        lf = LeafRef{brick, lidx};
                         return STEP_LEAF;
                     }
                 }
@@ -745,9 +766,8 @@ __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], 
             const int st = esvo_step<COUNT, STAGED, false, LOD>(s, r, cnt, lf);
             mode = st == STEP_DONE ? DONE : (st == STEP_LEAF ? LEAF : ESVO);
             if (mode == LEAF) { // parked in the lane's LDS column while it waits
-                *lane_word(s.aux_base, 11) = lf.brick;
-                *lane_word(s.aux_base, 12) = lf.cell;
-                *lane_word(s.aux_base, 13) = lf.lidx;
+                *lane_word(s, s.aux_base, 11) = lf.brick;
+                *lane_word(s, s.aux_base, 12) = lf.lidx;
             }
         }
         const uint64_t live = __ballot(mode != DONE);
@@ -755,9 +775,9 @@ __device__ __forceinline__ bool trace_ray(const SceneView& s, const float o[3], 
         const uint32_t waiting = (uint32_t)__popcll(__ballot(mode == LEAF));
         // finished lanes, still in the loop, and lanes still traversing stay out
         if (waiting * 4u >= (uint32_t)__popcll(live) * 3u && mode == LEAF) {
-            const uint32_t c = *lane_word(s.aux_base, 12);
-            const int bc[3] = {(int)(c << 22) >> 22, (int)(c << 12) >> 22, (int)(c << 2) >> 22}; // 10-bit signed
-            if (visit_leaf<COUNT>(s, r, *lane_word(s.aux_base, 11), bc, *lane_word(s.aux_base, 13), cnt))
+            int bc[3];
+            brick_cell(s, r, bc); // the ray state is unchanged while the leaf waits
+            if (visit_leaf<COUNT>(s, r, *lane_word(s, s.aux_base, 11), bc, *lane_word(s, s.aux_base, 12), cnt))
                 mode = DONE;
             else mode = resume_after_leaf(s, r) ? DONE : ESVO;
         }
