@@ -395,7 +395,7 @@ static SceneView alb_view(const vx_scene* s, uint32_t& lds)
     SceneView v = s->view;
     v.lds_off = 0;
     v.lds_count = 0;
-    lds = lay_out_lanes(v, 0, kAlbBlock);
+    lds = lay_out_lanes(v, 0, kAlbBlock, false);
     return v;
 }
 

@@ -566,15 +566,16 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     return VX_OK;
 }
 
-// The scene's view for a plain launch of kBlock lanes; lds receives the dynamic LDS bytes.
-static SceneView launch_view(const vx_scene* s, uint32_t& lds)
+// The scene's view for a plain launch of kBlock lanes (parks: the kernel
+// postpones leaf visits); lds receives the dynamic LDS bytes.
+static SceneView launch_view(const vx_scene* s, uint32_t& lds, bool parks)
 {
     SceneView v = s->view;
     v.lds_off = 0;
     v.lds_count = staged_nodes(s, s->lds_depth);
-    const uint32_t cap = (kMaxLdsBytes - kBlock * lane_lds_bytes(v.stack_depth)) / 8u;
+    const uint32_t cap = (kMaxLdsBytes - kBlock * lane_lds_bytes(v.stack_depth, parks)) / 8u;
     if (v.lds_count > cap) v.lds_count = cap;
-    lds = lay_out_lanes(v, v.lds_count, kBlock);
+    lds = lay_out_lanes(v, v.lds_count, kBlock, parks);
     return v;
 }
 
@@ -588,7 +589,7 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
     uint32_t lds = 0;
-    SceneView v = launch_view(s, lds);
+    SceneView v = launch_view(s, lds, true); // k_trace postpones leaf visits
     const bool staged = v.lds_count > 0;
     if (lod && lod->ray_dir_size > 0.0f) { // LODParameters::isEnabled
         v.lod_orig = lod->ray_orig_size;
@@ -746,7 +747,7 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint32_t lds = 0;
-    const vx::SceneView v = vx::launch_view(s, lds);
+    const vx::SceneView v = vx::launch_view(s, lds, false); // k_render visits leaves at once
     const dim3 grid(per * vx::kXcds), block(vx::kBlock);
     const int shade = !rgba8 ? vx::SHADE_NONE : ((v.nrm && v.bc1) ? vx::SHADE_COMPRESSED : vx::SHADE_FACE);
     auto launch = [&](auto kernel) {

@@ -196,7 +196,7 @@ static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tun
     // one per-lane layout for the launch: the deepest scene's stack
     uint32_t depth = 1;
     for (uint32_t s = 0; s < b->n; ++s) depth = std::max(depth, b->views[s].stack_depth);
-    const uint32_t lanes = t.waves_per_group * 64, lane_bytes = lanes * lane_lds_bytes(depth);
+    const uint32_t lanes = t.waves_per_group * 64, lane_bytes = lanes * lane_lds_bytes(depth, false);
     for (uint32_t s = 0; s < b->n; ++s) {
         if (off[s + 1] < off[s]) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: ray_offsets must be non-decreasing");
         BatchSceneInfo& e = tab[s];
@@ -214,7 +214,7 @@ static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tun
     }
     for (BatchSceneInfo& e : tab) {
         e.view.stack_depth = depth;
-        lay_out_lanes(e.view, lds_acc, lanes);
+        lay_out_lanes(e.view, lds_acc, lanes, false); // step_ray: leaves are visited at once
     }
     n_chunks = chunk_acc;
     lds_bytes = lds_acc * 8u + lane_bytes;

@@ -410,21 +410,28 @@ __device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
 }
 
 // Per-lane LDS words after the stack: the brick-DDA inputs ol[3], dl[3],
-// rinv[3], tstart, texit, then a pending leaf's brick id and leaf descriptor
-// index (its brick cell is recomputed from the unchanged ray state).
-constexpr uint32_t kAuxWords = 13;
+// rinv[3], tstart, texit, then (kernels that postpone leaf visits only) a
+// pending leaf's brick id and leaf descriptor index (its brick cell is
+// recomputed from the unchanged ray state).
+constexpr uint32_t kAuxWords = 11;
+constexpr uint32_t kParkWords = 2;
 // Words per lane column: stack (2 words per entry) and brick-DDA inputs; odd
-// for every depth since kAuxWords is odd.
-constexpr uint32_t lane_words(uint32_t stack_depth) { return 2u * stack_depth + kAuxWords; }
-static_assert(kAuxWords % 2u == 1u, "lane columns must be an odd number of words apart");
-constexpr uint32_t lane_lds_bytes(uint32_t stack_depth) { return lane_words(stack_depth) * 4u; }
-// Lays out the per-lane columns after `staged` descriptors; returns the kernel's LDS bytes.
-inline uint32_t lay_out_lanes(SceneView& v, uint32_t staged, uint32_t lanes)
+// for every depth, with or without the parked-leaf words.
+constexpr uint32_t lane_words(uint32_t stack_depth, bool parks)
+{
+    return 2u * stack_depth + kAuxWords + (parks ? kParkWords : 0u);
+}
+static_assert(kAuxWords % 2u == 1u && kParkWords % 2u == 0u, "lane columns must be an odd number of words apart");
+constexpr uint32_t lane_lds_bytes(uint32_t stack_depth, bool parks) { return lane_words(stack_depth, parks) * 4u; }
+// Lays out the per-lane columns after `staged` descriptors for a launch of
+// `lanes` lanes; parks: the kernel postpones leaf visits (trace_ray with
+// POSTPONE). Returns the kernel's LDS bytes.
+inline uint32_t lay_out_lanes(SceneView& v, uint32_t staged, uint32_t lanes, bool parks)
 {
     v.stack_base = staged * 2u;
     v.aux_base = 2u * v.stack_depth;
-    v.col_words = lane_words(v.stack_depth);
-    return staged * 8u + lanes * lane_lds_bytes(v.stack_depth);
+    v.col_words = lane_words(v.stack_depth, parks);
+    return staged * 8u + lanes * lane_lds_bytes(v.stack_depth, parks);
 }
 
 // Per-lane traversal state between iterations of the ESVO loop. Holding it in
