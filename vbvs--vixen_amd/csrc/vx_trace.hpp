@@ -16,13 +16,18 @@
 // Layout choices that are this backend's own:
 //   - the traversal stack only holds the scales the scene's tree can reach
 //     (22 down to one above the brick level: log2(bricks per axis) - 1 entries,
-//     at most VX_STACK) and lives in LDS, one lane-strided column per lane
-//     (bank-conflict free, no VGPRs, no scratch); every entry a ray reads was
-//     written earlier on its own path, so the stack needs no initialisation;
+//     at most VX_STACK) and lives in LDS, in a contiguous column per lane
+//     (columns an odd number of words apart: bank-conflict free, no VGPRs, no
+//     scratch); every entry a ray reads was written earlier on its own path,
+//     so the stack needs no initialisation;
 //   - the ray's brick-DDA inputs (local origin and direction, reciprocal
-//     direction, traced segment: 11 words) and a pending leaf (3 words) are
-//     needed only at brick visits, so they too live in lane-strided LDS
-//     columns instead of VGPRs across the ESVO loop (more waves per SIMD);
+//     direction, traced segment: 11 words) and, where visits are postponed, a
+//     pending leaf (2 words) are needed only at brick visits, so they too live
+//     in the lane's LDS column instead of VGPRs across the ESVO loop (more
+//     waves per SIMD); a finished ray's record goes there as well, so the loop
+//     carries no hit-record registers;
+//   - a per-brick box of occupied voxels, derived at upload, rejects brick
+//     segments that cannot hit before the DDA is set up;
 //   - for incoherent rays, brick visits are postponed until most live lanes of
 //     the wave wait at one, so the wave runs the DDA loop once for all of them
 //     (trace_ray);
