@@ -25,9 +25,12 @@ Also:
                 4*DDA voxels + 8*hits + 32 (record out), counted by the kernel's
                 own counting variant on the same frame; traffic = measured HBM
                 bytes per launch from the committed rocprofv3 PMC summary
+  launch_ms     the headline launch again, each bracketed by its own events:
+                median, min and max over max(20, --steps) launches
   cpu_baseline  the strict-FP32 C oracle on the host cores, same frame, all
-                cores (median of 5 after a warm-up): reported, not a target
-  parity        the last timed frame compared bit for bit with the oracle's
+                cores (median of 5 after a warm-up), plus a 1-thread figure on
+                200k rays of the frame and the CPU model: reported, not a target
+  parity        the last headline frame compared bit for bit with the oracle's
 
 --vixen-json PATH also writes the headline frames in the per-test results schema
 of VIXEN's benchmark runner (libraries/Profiler/src/MetricsExporter.cpp:117-230),
@@ -81,7 +84,17 @@ def timed(torch, stream, fn, steps: int, warmup: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def cpu_baseline(orc, built, rays, reps: int = 5):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(orc, built, rays, reps: int = 5, single_rays: int = 200_000):
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 64))
     scene = orc.OracleScene.from_built(built)
@@ -93,8 +106,17 @@ def cpu_baseline(orc, built, rays, reps: int = 5):
         out, _ = orc.trace(scene, rays, nthreads=threads)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    # one thread on an evenly spaced subset of the same frame
+    sub = np.ascontiguousarray(rays[:: max(1, len(rays) // single_rays)][:single_rays])
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.trace(scene, sub, nthreads=1)
+        t1.append(time.perf_counter() - t0)
     return {"value": len(rays) / med / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"the full {len(rays)}-ray frame, {threads} threads, median of {reps} after a warm-up"}, out
+            "sample": f"the full {len(rays)}-ray frame, {threads} threads, median of {reps} after a warm-up; "
+                      f"single_thread_value: {len(sub)} evenly spaced rays of it, 1 thread, median of 3",
+            "single_thread_value": len(sub) / float(np.median(t1)) / 1e6, "cpu_model": cpu_model()}, out
 
 
 # ---------------------------------------------------------------------------
@@ -338,6 +360,14 @@ def main() -> int:
     rays = W * H
     mrays = rays / (ms_step * 1e-3) / 1e6
     log(f"[bench] {args.steps} launches: {ms_step:.4f} ms/launch, {mrays:.1f} Mrays/s (wall {wall:.3f}s)")
+    # the same launch, each bracketed by its own events (median, as GPUTimestampQuery reports per frame)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(max(20, args.steps))]
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    per_launch = np.array([a.elapsed_time(b) for a, b in ev])
 
     if args.calib_copy_mib:
         src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev)
@@ -367,6 +397,9 @@ def main() -> int:
         "config": {"workload": WORKLOAD, "grid": N, "rays_per_step": rays, "width": W, "height": H,
                    "bricks": built.brick_count, "descriptors": built.node_count, "colour": "bc1"},
         "roofline": roofline,
+        "launch_ms": {"median": round(float(np.median(per_launch)), 5), "min": round(float(per_launch.min()), 5),
+                      "max": round(float(per_launch.max()), 5), "launches": len(per_launch),
+                      "median_mrays": round(rays / (float(np.median(per_launch)) * 1e-3) / 1e6, 1)},
     }
     got = vx.hits_to_numpy(hits)
     if not args.no_incoherent:
