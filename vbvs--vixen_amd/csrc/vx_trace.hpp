@@ -96,6 +96,17 @@ __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
 
+// Element i of a scene array through a 32-bit byte offset from its (uniform)
+// base, so the load takes the base in SGPRs and only a 32-bit offset per lane,
+// with no 64-bit address arithmetic. Valid while i * sizeof(T) < 2^32: the
+// scene checks bound the descriptor count below 2^29, and 24-bit brick ids keep
+// the occupancy, box and BC1 pools below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+
 __device__ __forceinline__ void store_miss(vx_hit& h, uint32_t iters)
 {
     h.flags = iters << 16;
@@ -236,7 +247,7 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         // occupied voxels' box, grown by 0.25 voxel, finds no hit: return what
         // the DDA would. (The counting variant keeps the full walk, so its
         // counts are the reference algorithm's.)
-        const uint32_t b = s.box[brick];
+        const uint32_t b = ld32(s.box, brick);
         float en = btmin, ex = btmax;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -274,9 +285,9 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     // mask word, always from inside this brick, before it tests the current one.
     // The step is written with selects (no divergent branches); the hit record is
     // evaluated once, after the loop.
-    const uint32_t* occ = s.occ + (size_t)brick * 16u;
+    const uint32_t occ = brick * 16u; // first mask word of the brick
     uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
-    uint32_t word = occ[lin >> 5];
+    uint32_t word = ld32(s.occ, occ + (lin >> 5));
     bool hit = false;
     uint32_t last = 0u; // axis of the step into the current voxel (1 x, 2 y, 3 z; 0 = entry voxel)
     for (int it = 0;; ++it) {
@@ -291,7 +302,7 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         const int n2 = cur[2] + (az ? step[2] : 0);
         const bool more = !(tsel > btmax) & (it + 1 < 24) & ((uint32_t)(n0 | n1 | n2) <= 7u);
         const uint32_t nlin = (uint32_t)((n0 & 7) + 8 * (n1 & 7) + 64 * (n2 & 7));
-        const uint32_t nword = occ[nlin >> 5];
+        const uint32_t nword = ld32(s.occ, occ + (nlin >> 5));
         const bool occupied = (word >> (lin & 31u)) & 1u;
         if (occupied | !more) {
             hit = occupied;
@@ -359,7 +370,7 @@ template <bool STAGED>
 __device__ __forceinline__ uint2 fetch_node(const SceneView& s, uint32_t i)
 {
     if (STAGED && i < s.lds_count) return vx_lds_nodes[s.lds_off + i];
-    return s.nodes[i];
+    return ld32(s.nodes, i);
 }
 
 __device__ __forceinline__ uint32_t* lane_word(const SceneView& s, uint32_t base, uint32_t j)
@@ -410,7 +421,7 @@ __device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
     h.brick = *lane_word(s, s.aux_base, 3);
     h.voxel = *lane_word(s, s.aux_base, 4);
     if (f & VX_HIT_FLAG_LOD) h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
-    else if (s.bc1) bc1_decode(s.bc1[(size_t)h.brick * 32u + (h.voxel >> 4)], h.voxel & 15u, h.rgb);
+    else if (s.bc1) bc1_decode(ld32(s.bc1, h.brick * 32u + (h.voxel >> 4)), h.voxel & 15u, h.rgb);
     else rgba8_decode(s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
 }
 
