@@ -152,3 +152,74 @@ def test_wider_cones_stop_sooner(vx, orc):
         assert (proj >= edge * (1 - 1e-6)).all()
         prev = lod
     assert fired[0] <= fired[1] <= fired[2] <= fired[3] and fired[3] > 0
+
+
+# The reference's LOD ray-casting scenarios (libraries/SVO/tests/test_lod.cpp:204-316),
+# one voxel in a cubic world of the reference's size and depth, on the oracle and,
+# as gpu-marked twins, through the HIP adapter's castRay*LOD methods.
+BACKENDS = ["oracle", pytest.param("hip", marks=pytest.mark.gpu)]
+
+
+class _LodCaster:
+    def __init__(self, vx, orc, backend, world, depth, voxel_pos):
+        N = 1 << depth
+        grid = np.zeros((N, N, N), np.uint32)
+        x, y, z = (int(c * N / world) for c in voxel_pos)  # world position -> voxel, as the reference bins it
+        grid[z, y, x] = 0xFFFFFFFF
+        self.built = vx.build_from_dense(grid, bc1=False)
+        for i in range(3):
+            self.built.config.grid_max[i] = world  # world [0, world]^3
+        self.vx, self.orc, self.backend = vx, orc, backend
+        if backend == "oracle":
+            self.scene = orc.OracleScene.from_built(self.built)
+        else:
+            self.ctx = vx.Context(0)
+            self.octree = vx.LaineKarrasOctreeHip(vx.Scene(self.ctx, self.built))
+
+    def cast(self, origin, direction, lod=None):
+        """(hit, t, user scale of an LOD hit or None)"""
+        if self.backend == "hip":
+            h = (self.octree.castRay(origin, direction) if lod is None
+                 else self.octree.castRayWithLOD(origin, direction, lod))
+            return h.hit, h.tMin, h.scale
+        r = np.zeros(1, self.orc.RAY_DTYPE)
+        r["o"], r["d"], r["tmax"] = origin, direction, 1e30
+        h = (self.orc.trace(self.scene, r)[0] if lod is None
+             else self.orc.trace_lod(self.scene, r, lod.ray_orig_size, lod.ray_dir_size))[0]
+        f = int(h["flags"])
+        return bool(f & 1), float(h["t"]), ((f >> 8) & 31) if f & (1 << 6) else None
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_reference_distant_voxel(vx, orc, backend):
+    # DistantVoxelTerminatesEarly: one voxel at (50,50,50) in [0,100]^3, depth 8, seen
+    # from 450 m with a 60 deg / 600-line camera. The cone at the grid (0.70 m) is
+    # below the 3.1 m brick edge, so the cast runs to full detail: same hit as castRay.
+    c = _LodCaster(vx, orc, backend, 100.0, 8, (50, 50, 50))
+    full = c.cast((50.0, 50.0, 500.0), (0.0, 0.0, -1.0))
+    lod = c.cast((50.0, 50.0, 500.0), (0.0, 0.0, -1.0), vx.lod_from_camera(math.radians(60.0), 600))
+    assert full[0] and lod == full
+    assert full[1] == pytest.approx(500.0 - 50.0 - 100.0 / 256, rel=1e-6)  # the voxel's near face
+    # a cone a hundred times wider stops at a coarse node before the brick
+    wide = c.cast((50.0, 50.0, 500.0), (0.0, 0.0, -1.0), vx.VxLod(0.0, 0.2))
+    assert wide[0] and wide[2] is not None and wide[1] <= full[1]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_reference_nearby_voxel(vx, orc, backend):
+    # NearbyVoxelReachesFullDetail: [0,10]^3, depth 6, voxel at (5,5,5), 3 m away, 1080 lines
+    c = _LodCaster(vx, orc, backend, 10.0, 6, (5, 5, 5))
+    full = c.cast((5.0, 5.0, 8.0), (0.0, 0.0, -1.0))
+    lod = c.cast((5.0, 5.0, 8.0), (0.0, 0.0, -1.0), vx.lod_from_camera(math.radians(60.0), 1080))
+    assert full[0] and lod == full
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_reference_explicit_and_disabled(vx, orc, backend):
+    # CastRayWithLODAcceptsExplicitParameters and DisabledLODMatchesRegularCast
+    c = _LodCaster(vx, orc, backend, 10.0, 6, (5, 5, 5))
+    full = c.cast((5.0, 5.0, 15.0), (0.0, 0.0, -1.0))
+    explicit = c.cast((5.0, 5.0, 15.0), (0.0, 0.0, -1.0), vx.VxLod(0.0, 0.01))
+    assert explicit[0]  # the cone (0.05 at the grid) is below the brick edge (1.25): full detail
+    assert explicit == full
+    assert c.cast((5.0, 5.0, 15.0), (0.0, 0.0, -1.0), vx.VxLod(0.0, 0.0)) == full

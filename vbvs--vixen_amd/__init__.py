@@ -35,6 +35,7 @@ STATUS_NAMES = {0: "VX_OK", 1: "VX_ERR_INVALID_ARG", 2: "VX_ERR_INVALID_SCENE", 
                 4: "VX_ERR_OUT_OF_MEMORY", 5: "VX_ERR_UNSUPPORTED"}
 
 HIT_FLAG = 1
+LOD_FLAG = 1 << 6  # VX_HIT_FLAG_LOD
 
 
 class VixenError(RuntimeError):
@@ -744,24 +745,39 @@ class RayHit:
     voxel: int
     leaf_node: int
     color: tuple[float, float, float]
+    scale: int | None = None  # user scale of an LOD hit (esvoToUserScale), None otherwise
 
 
 class LaineKarrasOctreeHip:
-    """castRay surface of LaineKarrasOctree (libraries/SVO/src/SVOTraversal.cpp:189-198) on the GPU."""
+    """castRay surface of LaineKarrasOctree (libraries/SVO/src/SVOTraversal.cpp:189-241) on the GPU."""
 
     def __init__(self, scene: Scene):
         self.scene = scene
 
-    def castRays(self, rays: np.ndarray) -> np.ndarray:
+    def castRays(self, rays: np.ndarray, lod: VxLod | None = None) -> np.ndarray:
         d = rays_to_device(np.asarray(rays, dtype=RAY_DTYPE), self.scene.ctx.device)
-        out = hits_to_numpy(self.scene.trace(d))
+        out = hits_to_numpy(self.scene.trace(d) if lod is None else self.scene.trace_lod(d, lod))
         _torch().cuda.synchronize()
         return out
 
     def castRay(self, origin, direction, tMin: float = 0.0, tMax: float = 1e30) -> RayHit:
+        return self._cast(origin, direction, tMin, tMax, None)
+
+    def castRayLOD(self, origin, direction, lodBias: float, tMin: float = 0.0, tMax: float = 1e30) -> RayHit:
+        """The legacy interface: lodBias is the cone growth per unit distance (SVOTraversal.cpp:198-210)."""
+        return self._cast(origin, direction, tMin, tMax, VxLod(0.0, lodBias))
+
+    def castRayScreenSpaceLOD(self, origin, direction, fovY: float, screenHeight: int, tMin: float = 0.0,
+                              tMax: float = 1e30) -> RayHit:
+        return self._cast(origin, direction, tMin, tMax, lod_from_camera(fovY, screenHeight))
+
+    def castRayWithLOD(self, origin, direction, lodParams: VxLod, tMin: float = 0.0, tMax: float = 1e30) -> RayHit:
+        return self._cast(origin, direction, tMin, tMax, lodParams)
+
+    def _cast(self, origin, direction, tMin, tMax, lod) -> RayHit:
         r = np.zeros(1, RAY_DTYPE)
         r["o"][0], r["d"][0], r["tmin"][0], r["tmax"][0] = origin, direction, tMin, tMax
-        h = self.castRays(r)[0]
+        h = self.castRays(r, lod)[0]
         hit = bool(h["flags"] & HIT_FLAG)
         o = np.asarray(origin, np.float64)
         dd = np.asarray(direction, np.float64)
@@ -772,8 +788,9 @@ class LaineKarrasOctreeHip:
             normal[ax] = 1.0 if (int(h["flags"]) >> 3) & 1 else -1.0
         t = float(h["t"])
         hp = tuple(o + dd * t) if hit else (math.nan, math.nan, math.nan)
+        scale = int(h["flags"] >> 8) & 31 if int(h["flags"]) & LOD_FLAG else None
         return RayHit(hit, t, hp, tuple(normal), int(h["brick"]), int(h["voxel"]), int(h["leaf_node"]),
-                      tuple(float(x) for x in h["rgb"]))
+                      tuple(float(x) for x in h["rgb"]), scale)
 
 
 def version() -> str:
