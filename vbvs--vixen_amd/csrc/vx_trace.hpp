@@ -201,8 +201,10 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
     for (int i = 0; i < 3; ++i) {
         const float t0 = (0.0f - ol[i]) * invd[i];
         const float t1 = (s.Nf - ol[i]) * invd[i];
-        tn[i] = mn(t0, t1);
-        tf[i] = mx(t0, t1);
+        // t0, t1: products of finite values (never NaN) and never both zero, so
+        // IEEE min/max equal the reference's ternaries here
+        tn[i] = fmn(t0, t1);
+        tf[i] = fmx(t0, t1);
     }
     float tEntry = mx(mx(tn[0], tn[1]), tn[2]);
     float tExit = mn(mn(tf[0], tf[1]), tf[2]);
@@ -232,8 +234,8 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         const float bmax = bmin[i] + 8.0f;
         const float t0 = (bmin[i] - ol[i]) * rinv[i];
         const float t1 = (bmax - ol[i]) * rinv[i];
-        tn[i] = mn(t0, t1);
-        tf[i] = mx(t0, t1);
+        tn[i] = fmn(t0, t1); // as in setup_ray: t0, t1 never NaN, never both zero
+        tf[i] = fmx(t0, t1);
     }
     float btmin = mx(mx(tn[0], tn[1]), tn[2]);
     float btmax = mn(mn(tf[0], tf[1]), tf[2]);
@@ -285,7 +287,14 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     // mask word, always from inside this brick, before it tests the current one.
     // The step is written with selects (no divergent branches); the hit record is
     // evaluated once, after the loop.
+    //
+    // The voxel is tracked as its linear index alone, lin = x + 8y + 64z. A step
+    // adds +-1, +-8 or +-64; it stays in the brick exactly when no bit outside
+    // the stepped axis's 3-bit field changes (a coordinate of 8 carries into the
+    // next field, one of -1 borrows from it; for z both leave [0, 512)), which
+    // is the reference's 0 <= x, y, z <= 7 test on the new coordinates.
     const uint32_t occ = brick * 16u; // first mask word of the brick
+    const uint32_t d0 = (uint32_t)step[0], d1 = (uint32_t)(8 * step[1]), d2 = (uint32_t)(64 * step[2]);
     uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
     uint32_t word = ld32(s.occ, occ + (lin >> 5));
     bool hit = false;
@@ -297,13 +306,12 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         const bool ay = !ax & (tNext[1] < tNext[2]);
         const bool az = !ax & !ay;
         const float tsel = ax ? tNext[0] : (ay ? tNext[1] : tNext[2]);
-        const int n0 = cur[0] + (ax ? step[0] : 0);
-        const int n1 = cur[1] + (ay ? step[1] : 0);
-        const int n2 = cur[2] + (az ? step[2] : 0);
-        const bool more = !(tsel > btmax) & (it + 1 < 24) & ((uint32_t)(n0 | n1 | n2) <= 7u);
-        const uint32_t nlin = (uint32_t)((n0 & 7) + 8 * (n1 & 7) + 64 * (n2 & 7));
-        const uint32_t nword = ld32(s.occ, occ + (nlin >> 5));
-        const bool occupied = (word >> (lin & 31u)) & 1u;
+        const uint32_t nlin = lin + (ax ? d0 : (ay ? d1 : d2));
+        const uint32_t field = ax ? 0x007u : (ay ? 0x038u : 0x1C0u);
+        const bool inside = ((nlin ^ lin) & ~field) == 0u;
+        const bool more = !(tsel > btmax) & (it + 1 < 24) & inside;
+        const uint32_t nword = ld32(s.occ, occ + ((nlin & 511u) >> 5));
+        const bool occupied = __builtin_amdgcn_ubfe(word, lin, 1u) != 0u; // bit lin mod 32
         if (occupied | !more) {
             hit = occupied;
             break;
@@ -312,14 +320,14 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         tNext[0] = ax ? s0 : tNext[0];
         tNext[1] = ay ? s1 : tNext[1];
         tNext[2] = az ? s2 : tNext[2];
-        cur[0] = n0;
-        cur[1] = n1;
-        cur[2] = n2;
         lin = nlin;
         word = nword;
         last = ax ? 1u : (ay ? 2u : 3u);
     }
     if (!hit) return false;
+    cur[0] = (int)(lin & 7u);
+    cur[1] = (int)((lin >> 3) & 7u);
+    cur[2] = (int)(lin >> 6);
 
     // Entry time of the hit voxel per axis: the reference takes min(t0, t1) of
     // its two faces; rounding is monotone, so that is the near face's quotient
