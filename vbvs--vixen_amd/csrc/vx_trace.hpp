@@ -106,6 +106,12 @@ __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i)
 {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
 }
+// The same with a byte offset.
+template <class T>
+__device__ __forceinline__ T ldb32(const T* __restrict__ base, uint32_t byte_offset)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_offset);
+}
 
 __device__ __forceinline__ void store_miss(vx_hit& h, uint32_t iters)
 {
@@ -217,15 +223,16 @@ __device__ __forceinline__ bool setup_ray(const SceneView& s, const float o[3], 
     return true;
 }
 
-// Brick DDA in local voxel space. Returns true on a hit, with the record's
-// hit/axis/sign/step bits in flags, the entry distance in t and the voxel.
+// Brick DDA in local voxel space. Returns true on a hit, with the hit voxel
+// (x + 8y + 64z) and the axis of the step into it (1 x, 2 y, 3 z; 0 when it is
+// the first voxel tested); hit_entry then gives the record's t and face.
 // rinv[i] is the ray's brick reciprocal, 1/dl[i] (+-1e8 when |dl[i]| < 1e-8),
 // computed once per ray: the reference's per-brick 1/dl and 1/|dl| are the
 // same correctly rounded values (RN(1/|x|) = |RN(1/x)|).
 template <bool COUNT>
 __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, const int bc[3], const float ol[3],
                                           const float dl[3], const float rinv[3], float tstart, float texit,
-                                          uint32_t& flags, float& t, uint32_t& voxel, TraceCounts& cnt)
+                                          uint32_t& voxel, uint32_t& step_axis, TraceCounts& cnt)
 {
     float bmin[3], tn[3], tf[3];
 #pragma unroll
@@ -289,14 +296,16 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
     // evaluated once, after the loop.
     //
     // The voxel is tracked as its linear index alone, lin = x + 8y + 64z. A step
-    // adds +-1, +-8 or +-64; it stays in the brick exactly when no bit outside
-    // the stepped axis's 3-bit field changes (a coordinate of 8 carries into the
-    // next field, one of -1 borrows from it; for z both leave [0, 512)), which
-    // is the reference's 0 <= x, y, z <= 7 test on the new coordinates.
-    const uint32_t occ = brick * 16u; // first mask word of the brick
+    // adds +-1, +-8 or +-64, which leaves the bits below the stepped axis's 3-bit
+    // field alone; it stays in the brick exactly when nothing above the field
+    // changes either (a coordinate of 8 carries out of the field, one of -1
+    // borrows), i.e. when lin ^ nlin < 8, 64 or 512 for an x, y or z step: the
+    // reference's 0 <= x, y, z <= 7 test on the new coordinates.
+    const uint32_t occ = brick * 64u; // byte offset of the brick's 16 mask words (64-byte aligned)
     const uint32_t d0 = (uint32_t)step[0], d1 = (uint32_t)(8 * step[1]), d2 = (uint32_t)(64 * step[2]);
     uint32_t lin = (uint32_t)(cur[0] + 8 * cur[1] + 64 * cur[2]);
-    uint32_t word = ld32(s.occ, occ + (lin >> 5));
+    // mask word of voxel v: byte occ + 4 * (v >> 5) = occ | ((v >> 3) & 60) (v < 512)
+    uint32_t word = ldb32(s.occ, occ | ((lin >> 3) & 60u));
     bool hit = false;
     uint32_t last = 0u; // axis of the step into the current voxel (1 x, 2 y, 3 z; 0 = entry voxel)
     for (int it = 0;; ++it) {
@@ -307,10 +316,9 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         const bool az = !ax & !ay;
         const float tsel = ax ? tNext[0] : (ay ? tNext[1] : tNext[2]);
         const uint32_t nlin = lin + (ax ? d0 : (ay ? d1 : d2));
-        const uint32_t field = ax ? 0x007u : (ay ? 0x038u : 0x1C0u);
-        const bool inside = ((nlin ^ lin) & ~field) == 0u;
+        const bool inside = (nlin ^ lin) < (ax ? 8u : (ay ? 64u : 512u));
         const bool more = !(tsel > btmax) & (it + 1 < 24) & inside;
-        const uint32_t nword = ld32(s.occ, occ + ((nlin & 511u) >> 5));
+        const uint32_t nword = ldb32(s.occ, occ | ((nlin >> 3) & 60u)); // in this brick even when nlin is not
         const bool occupied = __builtin_amdgcn_ubfe(word, lin, 1u) != 0u; // bit lin mod 32
         if (occupied | !more) {
             hit = occupied;
@@ -325,17 +333,25 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         last = ax ? 1u : (ay ? 2u : 3u);
     }
     if (!hit) return false;
-    cur[0] = (int)(lin & 7u);
-    cur[1] = (int)((lin >> 3) & 7u);
-    cur[2] = (int)(lin >> 6);
+    voxel = lin;
+    step_axis = last;
+    return true;
+}
 
+// The hit record's entry time and face for hit voxel `voxel` of the brick at
+// grid cell bc (SVOBrickDDA.cpp:470-502), flags = hit | axis << 1 | sign << 3
+// | step axis << 4.
+__device__ __forceinline__ void hit_entry(const int bc[3], uint32_t voxel, uint32_t step_axis, const float ol[3],
+                                          const float dl[3], float tstart, uint32_t& flags, float& t)
+{
+    const int cur[3] = {(int)(voxel & 7u), (int)((voxel >> 3) & 7u), (int)(voxel >> 6)};
     // Entry time of the hit voxel per axis: the reference takes min(t0, t1) of
     // its two faces; rounding is monotone, so that is the near face's quotient
     // ((vmin - o)/d for d > 0, (vmax - o)/d for d < 0): one division per axis.
     float tnh[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float vmin = bmin[i] + (float)cur[i];
+        const float vmin = (float)(bc[i] * 8) + (float)cur[i];
         const float vmax = vmin + 1.0f;
         const float tq = ((dl[i] > 0.0f ? vmin : vmax) - ol[i]) / dl[i];
         tnh[i] = __builtin_fabsf(dl[i]) < 1e-8f ? -__builtin_inff() : tq;
@@ -356,10 +372,8 @@ __device__ __forceinline__ bool brick_dda(const SceneView& s, uint32_t brick, co
         axis = 2u;
         signpos = sz;
     }
-    flags = 1u | (axis << 1) | (signpos << 3) | (last << 4);
+    flags = 1u | (axis << 1) | (signpos << 3) | (step_axis << 4);
     t = tHit;
-    voxel = lin;
-    return true;
 }
 
 // Descriptor fetch: a prefix of the descriptor array (the top levels of a
@@ -653,9 +667,11 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
         rinv[i] = u2f(*lane_word(s, s.aux_base, 6 + i));
     }
     const float tstart = u2f(*lane_word(s, s.aux_base, 9)), texit = u2f(*lane_word(s, s.aux_base, 10));
-    uint32_t flags, voxel;
+    uint32_t voxel, step_axis;
+    if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, voxel, step_axis, cnt)) return false;
+    uint32_t flags;
     float t;
-    if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, flags, t, voxel, cnt)) return false;
+    hit_entry(bc, voxel, step_axis, ol, dl, tstart, flags, t);
     rec_hit(s, flags | ((uint32_t)(r.iter + 1) << 16), t, lidx, brick, voxel);
     if (COUNT) cnt.hits += 1;
     return true;
