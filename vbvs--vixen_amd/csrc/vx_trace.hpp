@@ -88,10 +88,11 @@ struct TraceCounts {
 
 __device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
-// One-instruction IEEE min/max. They equal mn/mx whenever the operands are not
-// NaN and not a (-0, +0) pair; every use states why that holds there.
-__device__ __forceinline__ float fmn(float a, float b) { return __builtin_fminf(a, b); }
-__device__ __forceinline__ float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
+// One-instruction min/max (IEEE 754-2019 minimum/maximum: v_minimum3_f32 /
+// v_maximum3_f32, no operand quieting needed). They equal mn/mx whenever the
+// operands are not NaN and not a (-0, +0) pair; every use states why that holds.
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
@@ -669,9 +670,18 @@ __device__ __forceinline__ bool visit_leaf(const SceneView& s, const RayState& r
     const float tstart = u2f(*lane_word(s, s.aux_base, 9)), texit = u2f(*lane_word(s, s.aux_base, 10));
     uint32_t voxel, step_axis;
     if (!brick_dda<COUNT>(s, brick, bc, ol, dl, rinv, tstart, texit, voxel, step_axis, cnt)) return false;
+    // reload the ray inputs for the record (the barrier keeps the compiler from
+    // holding their per-axis tests live across the walk)
+    asm volatile("" ::: "memory");
+    float hol[3], hdl[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        hol[i] = u2f(*lane_word(s, s.aux_base, i));
+        hdl[i] = u2f(*lane_word(s, s.aux_base, 3 + i));
+    }
     uint32_t flags;
     float t;
-    hit_entry(bc, voxel, step_axis, ol, dl, tstart, flags, t);
+    hit_entry(bc, voxel, step_axis, hol, hdl, tstart, flags, t);
     rec_hit(s, flags | ((uint32_t)(r.iter + 1) << 16), t, lidx, brick, voxel);
     if (COUNT) cnt.hits += 1;
     return true;
