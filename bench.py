@@ -19,6 +19,11 @@ Two more legs ride in the same JSON line:
   incoherent  vx_trace on the same scene with incoherent rays: 2M seeded
             random rays from inside the volume, and one seeded cosine bounce
             ray per primary hit of the headline frame.
+  rows      the rows either side of the path on the same scene: the VSCA v1
+            scene cache written and read back (host), the compressed
+            shader's RGBA8 image (with the DXT normal pool), and screen-space
+            LOD on the frame's rays at two cone widths; each output compared
+            with the oracle's unless --no-cpu.
 Also:
   roofline      algorithmic bytes per launch / average launch time against the
                 8 TB/s HBM3E peak; per ray: 8*iterations + 8*brick visits +
@@ -285,6 +290,74 @@ def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int):
 
 
 # ---------------------------------------------------------------------------
+# the rows either side of the path
+# ---------------------------------------------------------------------------
+def rows_leg(vx, orc, torch, ctx, stream, built, cam, W, H, steps: int, check: bool):
+    """Scene ingest (VSCA v1 write + read), the compressed shader's RGBA8 image and screen-space LOD, on the
+    headline scene and frame. check: compare the GPU outputs with the oracle's."""
+    import tempfile
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        f = Path(d) / "cave256.vsca"
+        t0 = time.perf_counter()
+        vx.write_cache(f, [dict(built=built, scene_type="tunnels", resolution=256, density=0.5, seed=2)])
+        t_w = time.perf_counter() - t0
+        size = f.stat().st_size
+        t0 = time.perf_counter()
+        (_, rb), = vx.read_cache(f)
+        t_r = time.perf_counter() - t0
+    same = all(getattr(rb, k).tobytes() == getattr(built, k).tobytes() for k in ("nodes", "brick_data", "bc1"))
+    out["ingest"] = {"workload": "cave256 scene as a VSCA v1 cache file (host)", "file_mb": round(size / 1e6, 2),
+                     "write_ms": round(t_w * 1e3, 2), "read_ms": round(t_r * 1e3, 2),
+                     "read_gbps": round(size / t_r / 1e9, 2), "byte_identical": bool(same)}
+
+    t0 = time.perf_counter()
+    built.encode_normals()
+    t_n = time.perf_counter() - t0
+    shaded = vx.Scene(ctx, built)
+    rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
+
+    def frame():
+        vx._check(vx.LIB.vx_render(shaded._h, vx.C.byref(cam), W, H, 1e30, None, rgba.data_ptr(),
+                                   stream.cuda_stream), "vx_render")
+
+    ms = timed(torch, stream, frame, steps, 2)
+    shading = {"workload": "cave256 with its DXT normal pool, 1920x1080 RGBA8 image (compressed shader: BC1 colour, "
+                           "normal blend, light, sky)", "normal_encode_s": round(t_n, 2), "ms": round(ms, 4),
+               "mrays": round(W * H / ms / 1e3, 1)}
+    if check:
+        want = orc.render(orc.OracleScene.from_built(built), cam.as_floats(), W, H, 1e30, nthreads=16)
+        got = rgba.cpu().numpy().view(np.uint32)
+        shading["parity"] = "bit-exact vs oracle" if np.array_equal(got, want) else \
+            f"{int((got != want).sum())} pixels differ from the oracle"
+    out["shading"] = shading
+    shaded.close()
+
+    scene = vx.Scene(ctx, built)
+    rays = vx.camera_rays(cam, W, H, 1e30)
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    base = vx.lod_from_camera(math.radians(cam.fov_deg), H)
+    ms_plain = timed(torch, stream, lambda: scene.trace(rays, hits, stream=stream), steps, 2)
+    lod = {"workload": "cave256, the 1080p frame's rays, castRayScreenSpaceLOD (70 deg, 1080 lines) with a bias",
+           "trace_ms": round(ms_plain, 4)}
+    host_rays = rays.cpu().numpy().view(orc.RAY_DTYPE) if check else None
+    for bias in (0.0, 6.0):
+        p = vx.lod_with_bias(base, bias)
+        ms_lod = timed(torch, stream, lambda: scene.trace_lod(rays, p, hits, stream=stream), steps, 2)
+        got = vx.hits_to_numpy(hits)
+        row = {"ms": round(ms_lod, 4), "mrays": round(W * H / ms_lod / 1e3, 1),
+               "lod_stops": round(float(((got["flags"] >> 6) & 1).mean()), 4)}
+        if check:
+            want = orc.trace_lod(orc.OracleScene.from_built(built), host_rays, p.ray_orig_size, p.ray_dir_size,
+                                 nthreads=16)
+            row["parity"] = "bit-exact vs oracle" if got.tobytes() == want.tobytes() else "differs from the oracle"
+        lod[f"bias_{int(bias)}"] = row
+    out["lod"] = lod
+    scene.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
 # VIXEN results schema
 # ---------------------------------------------------------------------------
 def vixen_report(frame_ms, rays, read_bytes, write_bytes, iters_per_ray, voxel_res, scene_type, width, height,
@@ -323,6 +396,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent-ray leg")
+    ap.add_argument("--no-rows", action="store_true", help="skip the ingest/shading/LOD leg")
     ap.add_argument("--vixen-json", default=None, help="also write the headline frames in VIXEN's results schema")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -410,6 +484,11 @@ def main() -> int:
         log("[bench] fwd+bwd leg...")
         result["fwd_bwd"] = fwd_bwd_leg(vx, torch, ctx, stream, grid, built, max(3, args.steps // 5))
         log(f"[bench] fwd_bwd: {result['fwd_bwd']}")
+    if not args.no_rows:
+        log("[bench] ingest/shading/LOD leg...")
+        result["rows"] = rows_leg(vx, orc, torch, ctx, stream, built, cam, W, H, max(5, args.steps // 5),
+                                  check=not args.no_cpu)
+        log(f"[bench] rows: {result['rows']}")
     scene.close()
     if not args.no_batched:
         log("[bench] batched leg...")
