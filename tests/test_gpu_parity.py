@@ -199,6 +199,34 @@ def test_render_rgba_compressed_shading_bit_exact(vx, orc, cornell_bc1, cave, gp
     assert (img != face).mean() > 0.05
 
 
+@pytest.mark.parametrize("lds_depth", [1, 2, 4])
+def test_render_with_staged_descriptors_bit_exact(vx, orc, cave, gpu_ctx, lds_depth):
+    """vx_render's LDS-staged variants (the top breadth-first levels read from LDS):
+    hit records, entry-face RGBA8 and compressed-shader RGBA8, all against the oracle."""
+    grid, cam, built = cave
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    W, H = 640, 360
+    rays = orc.camera_rays(cam.as_floats(), W, H, 1e30)
+    osc = orc.OracleScene.from_built(b)
+    want_hits, _ = orc.trace(osc, rays, nthreads=16)
+    want_img = orc.render(osc, cam.as_floats(), W, H, 1e30, nthreads=16)
+    want_face = orc.render(orc.OracleScene.from_built(built), cam.as_floats(), W, H, 1e30, nthreads=16)
+    print(f"staged render lds_depth={lds_depth}", flush=True)
+    for scene_built, want_rgba, label in ((b, want_img, "compressed"), (built, want_face, "entry face")):
+        scene = vx.Scene(gpu_ctx, scene_built)
+        scene.set_lds_depth(lds_depth)
+        hits, img = scene.render(cam, W, H, tmax=1e30, hits=True, rgba=True)
+        _sync()
+        _assert_same(vx.hits_to_numpy(hits), want_hits, rays, f"staged {lds_depth} {label} hits")
+        got = img.cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, want_rgba), f"staged {lds_depth} {label}: {(got != want_rgba).sum()} pixels differ"
+        hits_only, _ = scene.render(cam, W, H, tmax=1e30, hits=True, rgba=False)
+        _sync()
+        _assert_same(vx.hits_to_numpy(hits_only), want_hits, rays, f"staged {lds_depth} hits only")
+        scene.close()
+
+
 def test_scene_from_cache_file_renders_like_built(vx, orc, cave, gpu_ctx, tmp_path):
     # a VSCA v1 cache round trip (VIXEN's scene cache format) feeds vx_scene_create unchanged
     grid, cam, built = cave
