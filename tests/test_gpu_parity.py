@@ -225,6 +225,18 @@ def test_render_with_staged_descriptors_bit_exact(vx, orc, cave, gpu_ctx, lds_de
         _sync()
         _assert_same(vx.hits_to_numpy(hits_only), want_hits, rays, f"staged {lds_depth} hits only")
         scene.close()
+    # vx_trace and vx_trace_counters read the same staged prefix
+    scene = vx.Scene(gpu_ctx, built)
+    scene.set_lds_depth(lds_depth)
+    d_rays = vx.rays_to_device(rays)
+    traced = vx.hits_to_numpy(scene.trace(d_rays))
+    counted, counters = scene.trace_counters(d_rays)
+    _sync()
+    _assert_same(traced, want_hits, rays, f"staged {lds_depth} vx_trace")
+    _assert_same(vx.hits_to_numpy(counted), want_hits, rays, f"staged {lds_depth} vx_trace_counters")
+    _, counts = orc.trace(osc, rays, nthreads=16)
+    assert counters.cpu().numpy()[:5].tolist() == counts.astype(np.int64).tolist()
+    scene.close()
 
 
 def test_scene_from_cache_file_renders_like_built(vx, orc, cave, gpu_ctx, tmp_path):
