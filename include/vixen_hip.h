@@ -252,8 +252,10 @@ vx_status vx_trace_lod(vx_scene* scene, const vx_ray* rays, uint64_t n, const vx
  * counters[0] rays, [1] ESVO iterations (8-byte descriptor fetch each),
  * [2] brick visits (8-byte leaf descriptor fetch each), [3] DDA voxels examined
  * (4-byte occupancy read each), [4] hits (8-byte BC1 block or 4-byte RGBA8 read).
- * Runs the same traversal as vx_trace with counting enabled; counters is a
- * device pointer to 8 u64 that the call zeroes first. */
+ * Runs the same traversal as vx_trace with counting enabled, without the
+ * occupied-box pretest vx_trace uses to skip brick walks that cannot hit, so
+ * the counts are the reference algorithm's (hit records are identical either
+ * way); counters is a device pointer to 8 u64 that the call zeroes first. */
 vx_status vx_trace_counters(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits,
                             uint64_t* counters, vx_stream stream);
 
