@@ -172,6 +172,18 @@ vx_status vx_ctx_destroy(vx_ctx* ctx);
  * 15-bit childPointer (the contour field is unused on the voxel path). */
 vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out);
 
+/* vx_scene_create without the upload: esvo_nodes, brick_data, bc1_colors and
+ * dxt_normals in desc are DEVICE pointers on the context's device (8-byte
+ * aligned descriptors and BC1 blocks, 16-byte aligned brick data and normal
+ * blocks; config stays a host pointer). The scene uses them in place, so the
+ * caller keeps them alive and unchanged until vx_scene_destroy; only the
+ * derived occupancy mask and boxes (68 B per brick) are allocated. The
+ * descriptors are copied to the host once for the tree checks. The device
+ * form of the scene buffers that the cacher uploads
+ * (VoxelSceneCacher::UploadToGPU, libraries/CashSystem/src/VoxelSceneCacher.cpp:719-891),
+ * e.g. torch tensors. */
+vx_status vx_scene_create_device(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out);
+
 /* The checks vx_scene_create runs before uploading, without touching a device. */
 vx_status vx_scene_validate(const vx_scene_desc* desc);
 vx_status vx_scene_destroy(vx_scene* scene);

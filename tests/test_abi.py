@@ -140,6 +140,8 @@ def test_trace_rejects_null_arguments_without_device(vx):
     assert vx.LIB.vx_trace(None, None, 10, None, None) == 1
     assert vx.LIB.vx_render(None, None, 4, 4, 1.0, None, None, None) == 1
     assert vx.LIB.vx_camera_rays(None, 4, 4, 1.0, None, None) == 1
+    out = C.c_void_p()
+    assert vx.LIB.vx_scene_create_device(None, None, C.byref(out)) == 1
 
 
 def test_builder_rejects_bad_sizes(vx):

@@ -239,6 +239,34 @@ def test_render_with_staged_descriptors_bit_exact(vx, orc, cave, gpu_ctx, lds_de
     scene.close()
 
 
+def test_scene_over_device_buffers_renders_like_copied(vx, orc, cave, gpu_ctx):
+    """vx_scene_create_device: the scene buffers as CUDA tensors, used in place."""
+    import torch
+    grid, cam, built = cave
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    dev = torch.device("cuda", 0)
+    t = {k: torch.from_numpy(getattr(b, k).view(np.uint32).copy()).to(dev) for k in ("nodes", "brick_data", "bc1",
+                                                                                    "normals")}
+    scene = vx.Scene.from_device(gpu_ctx, t["nodes"], t["brick_data"], b.config, bc1=t["bc1"], normals=t["normals"])
+    ref = vx.Scene(gpu_ctx, b)
+    h1, i1 = scene.render(cam, 640, 360, tmax=1e30, rgba=True)
+    h2, i2 = ref.render(cam, 640, 360, tmax=1e30, rgba=True)
+    _sync()
+    assert vx.hits_to_numpy(h1).tobytes() == vx.hits_to_numpy(h2).tobytes()
+    assert i1.cpu().numpy().tobytes() == i2.cpu().numpy().tobytes()
+    # a batch copies the borrowed pools, so it outlives the scene
+    batch = vx.Batch(gpu_ctx, [scene])
+    scene.close()
+    rays = vx.camera_rays(cam, 640, 360, 1e30)
+    got = batch.trace(rays, np.array([0, 640 * 360], np.uint64))
+    _sync()
+    assert vx.hits_to_numpy(got).tobytes() == vx.hits_to_numpy(h2).tobytes()
+    # misaligned brick words are refused before any device access
+    with pytest.raises(vx.VixenError):
+        vx.Scene.from_device(gpu_ctx, t["nodes"], t["brick_data"][1:], b.config)
+
+
 def test_scene_from_cache_file_renders_like_built(vx, orc, cave, gpu_ctx, tmp_path):
     # a VSCA v1 cache round trip (VIXEN's scene cache format) feeds vx_scene_create unchanged
     grid, cam, built = cave

@@ -144,6 +144,7 @@ def _load() -> C.CDLL:
         "vx_ctx_destroy": ([vp], st),
         "vx_scene_create": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_validate": ([C.POINTER(VxSceneDesc)], st),
+        "vx_scene_create_device": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_destroy": ([vp], st),
         "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
         "vx_trace": ([vp, vp, u64, vp, vp], st),
@@ -199,7 +200,7 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
             "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
             "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
-            "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias")
+            "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device")
 
 
 def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:
@@ -508,6 +509,29 @@ class Scene:
         h = C.c_void_p()
         _check(LIB.vx_scene_create(ctx._h, C.byref(desc), C.byref(h)), "vx_scene_create")
         self._h = h
+
+    @classmethod
+    def from_device(cls, ctx: Context, nodes, brick_data, config: VxOctreeConfig, bc1=None, normals=None) -> "Scene":
+        """A scene over caller-owned device buffers (vx_scene_create_device), e.g. CUDA tensors holding the
+        descriptor pairs, brick words, BC1 blocks and DXT normal blocks; they are used in place, so they are
+        kept referenced by the scene."""
+        self = cls.__new__(cls)
+        self.ctx, self.built = ctx, None
+        self._cfg = VxOctreeConfig()
+        C.memmove(C.addressof(self._cfg), C.addressof(config), C.sizeof(config))
+        self._keep = (nodes, brick_data, bc1, normals)
+        desc = VxSceneDesc()
+        desc.esvo_nodes = nodes.data_ptr()
+        desc.node_count = nodes.numel() * nodes.element_size() // 8
+        desc.brick_data = brick_data.data_ptr() if brick_data.numel() else None
+        desc.brick_count = brick_data.numel() * brick_data.element_size() // 2048
+        desc.bc1_colors = bc1.data_ptr() if bc1 is not None else None
+        desc.dxt_normals = normals.data_ptr() if normals is not None else None
+        desc.config = C.pointer(self._cfg)
+        h = C.c_void_p()
+        _check(LIB.vx_scene_create_device(ctx._h, C.byref(desc), C.byref(h)), "vx_scene_create_device")
+        self._h = h
+        return self
 
     def close(self) -> None:
         if self._h:

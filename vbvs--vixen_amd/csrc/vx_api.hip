@@ -501,12 +501,35 @@ static vx_status scene_validate_impl(const vx_scene_desc* d, uint32_t level_pref
     return validate_tree(d, levels, level_prefix);
 }
 
-static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out)
+// device: the buffers in d are device pointers the scene borrows (the config
+// stays a host pointer); otherwise host pointers the scene copies.
+static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** out, bool device)
 {
     if (!ctx || !d || !out) return fail(VX_ERR_INVALID_ARG, "vx_scene_create: null argument");
     *out = nullptr;
     uint32_t level_prefix[4];
-    if (vx_status st = scene_validate_impl(d, level_prefix); st != VX_OK) return st;
+    if (!device) {
+        if (vx_status st = scene_validate_impl(d, level_prefix); st != VX_OK) return st;
+    } else {
+        if (!d->esvo_nodes || !d->config) return fail(VX_ERR_INVALID_ARG, "vx_scene_create_device: null argument");
+        auto misaligned = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) != 0; };
+        if (misaligned(d->esvo_nodes, 8) || misaligned(d->brick_data, 16) || misaligned(d->bc1_colors, 8) ||
+            misaligned(d->dxt_normals, 16))
+            return fail(VX_ERR_INVALID_ARG, "vx_scene_create_device: descriptors, BC1 blocks need 8-byte and "
+                                            "brick data, normal blocks 16-byte alignment");
+        if (d->node_count < 1 || d->node_count >= (1ull << 29))
+            return fail(VX_ERR_INVALID_SCENE, "node_count out of range");
+        // the tree checks read the descriptors: one copy of them to the host
+        std::vector<uint32_t> host(d->node_count * 2);
+        int dev = 0;
+        VX_HIP(hipGetDevice(&dev));
+        if (dev != ctx->device) VX_HIP(hipSetDevice(ctx->device));
+        if (hipMemcpy(host.data(), d->esvo_nodes, d->node_count * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(VX_ERR_INVALID_ARG, "vx_scene_create_device: esvo_nodes is not readable device memory");
+        vx_scene_desc hd = *d;
+        hd.esvo_nodes = host.data();
+        if (vx_status st = scene_validate_impl(&hd, level_prefix); st != VX_OK) return st;
+    }
     const vx_octree_config& c = *d->config;
     const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
     const float sx = c.grid_max[0] - c.grid_min[0];
@@ -521,14 +544,17 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     s->node_count = d->node_count;
     s->brick_count = d->brick_count;
     s->cfg = c;
+    s->borrowed = device;
     std::memcpy(s->level_prefix, level_prefix, sizeof level_prefix);
     auto cleanup = [&](vx_status st) {
-        (void)hipFree(s->nodes);
-        (void)hipFree(s->brick_data);
+        if (!s->borrowed) {
+            (void)hipFree(s->nodes);
+            (void)hipFree(s->brick_data);
+            (void)hipFree(s->bc1);
+            (void)hipFree(s->nrm);
+        }
         (void)hipFree(s->occ);
         (void)hipFree(s->box);
-        (void)hipFree(s->bc1);
-        (void)hipFree(s->nrm);
         delete s;
         return st;
     };
@@ -536,16 +562,24 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     const size_t node_bytes = d->node_count * 8, brick_bytes = std::max<uint64_t>(1, nb) * 2048,
                  occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256,
                  nrm_bytes = std::max<uint64_t>(1, nb) * 512;
-    if (hipMalloc(&s->nodes, node_bytes) != hipSuccess || hipMalloc(&s->brick_data, brick_bytes) != hipSuccess ||
-        hipMalloc(&s->occ, occ_bytes) != hipSuccess ||
-        hipMalloc(&s->box, std::max<uint64_t>(1, nb) * 4) != hipSuccess || (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess) ||
-        (d->dxt_normals && hipMalloc(&s->nrm, nrm_bytes) != hipSuccess))
+    if (hipMalloc(&s->occ, occ_bytes) != hipSuccess || hipMalloc(&s->box, std::max<uint64_t>(1, nb) * 4) != hipSuccess)
         return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
-    if (hipMemcpy(s->nodes, d->esvo_nodes, node_bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        (nb && hipMemcpy(s->brick_data, d->brick_data, nb * 2048, hipMemcpyHostToDevice) != hipSuccess) ||
-        (nb && d->bc1_colors && hipMemcpy(s->bc1, d->bc1_colors, nb * 256, hipMemcpyHostToDevice) != hipSuccess) ||
-        (nb && d->dxt_normals && hipMemcpy(s->nrm, d->dxt_normals, nb * 512, hipMemcpyHostToDevice) != hipSuccess))
-        return cleanup(fail(VX_ERR_HIP, "vx_scene_create: upload failed"));
+    if (device) {
+        s->nodes = const_cast<uint2*>(reinterpret_cast<const uint2*>(d->esvo_nodes));
+        s->brick_data = const_cast<uint32_t*>(d->brick_data);
+        s->bc1 = const_cast<uint2*>(reinterpret_cast<const uint2*>(d->bc1_colors));
+        s->nrm = const_cast<uint4*>(reinterpret_cast<const uint4*>(d->dxt_normals));
+    } else {
+        if (hipMalloc(&s->nodes, node_bytes) != hipSuccess || hipMalloc(&s->brick_data, brick_bytes) != hipSuccess ||
+            (d->bc1_colors && hipMalloc(&s->bc1, bc1_bytes) != hipSuccess) ||
+            (d->dxt_normals && hipMalloc(&s->nrm, nrm_bytes) != hipSuccess))
+            return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
+        if (hipMemcpy(s->nodes, d->esvo_nodes, node_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            (nb && hipMemcpy(s->brick_data, d->brick_data, nb * 2048, hipMemcpyHostToDevice) != hipSuccess) ||
+            (nb && d->bc1_colors && hipMemcpy(s->bc1, d->bc1_colors, nb * 256, hipMemcpyHostToDevice) != hipSuccess) ||
+            (nb && d->dxt_normals && hipMemcpy(s->nrm, d->dxt_normals, nb * 512, hipMemcpyHostToDevice) != hipSuccess))
+            return cleanup(fail(VX_ERR_HIP, "vx_scene_create: upload failed"));
+    }
     if (nb) {
         const uint64_t words = nb * 16;
         k_build_occ<<<dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0>>>(s->brick_data, words, s->occ);
@@ -669,11 +703,22 @@ vx_status vx_ctx_destroy(vx_ctx* ctx)
 vx_status vx_scene_create(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out)
 {
     try {
-        return vx::scene_create(ctx, desc, out);
+        return vx::scene_create(ctx, desc, out, false);
     } catch (const std::bad_alloc&) {
         return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
     } catch (...) {
         return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_create: unexpected exception");
+    }
+}
+
+vx_status vx_scene_create_device(vx_ctx* ctx, const vx_scene_desc* desc, vx_scene** out)
+{
+    try {
+        return vx::scene_create(ctx, desc, out, true);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create_device: out of memory");
+    } catch (...) {
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_create_device: unexpected exception");
     }
 }
 
@@ -691,12 +736,14 @@ vx_status vx_scene_destroy(vx_scene* s)
 {
     if (!s) return VX_OK;
     vx_albedo_state_free(s->albedo);
-    (void)hipFree(s->nodes);
-    (void)hipFree(s->brick_data);
+    if (!s->borrowed) {
+        (void)hipFree(s->nodes);
+        (void)hipFree(s->brick_data);
+        (void)hipFree(s->bc1);
+        (void)hipFree(s->nrm);
+    }
     (void)hipFree(s->occ);
     (void)hipFree(s->box);
-    (void)hipFree(s->bc1);
-    (void)hipFree(s->nrm);
     delete s;
     return VX_OK;
 }

@@ -36,6 +36,9 @@ struct vx_scene {
     uint32_t level_prefix[4];
     uint32_t lds_depth; // levels staged in LDS by vx_trace / vx_render (0 = none)
     vx_albedo_state* albedo = nullptr; // differentiable-rendering state, built on first use
+    // vx_scene_create_device: nodes, brick_data, bc1 and nrm are the caller's
+    // device buffers (not freed here); occ and box are always the scene's own
+    bool borrowed = false;
 };
 
 namespace vx {
