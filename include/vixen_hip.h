@@ -342,7 +342,9 @@ vx_status vx_albedo_forward(vx_scene* scene, const vx_camera* camera, uint32_t w
  * hit the row. hit_row/hit_shade from vx_albedo_forward, or NULL to re-trace the
  * hits. mode VX_GRAD_ATOMIC (FP32 atomics) or VX_GRAD_DETERMINISTIC (stable sort
  * by row, then each row's run summed in a fixed order: identical bits on every
- * run; needs width*height < 2^31). grad_albedo accumulates. */
+ * run; needs width*height < 2^31). grad_albedo accumulates. The scene holds the
+ * albedo calls' row table and sort workspace (built and grown on first use), so
+ * albedo calls on one scene must not overlap; tracing and rendering only read it. */
 vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
                              const float light[3], const float* grad_image, const uint32_t* hit_row,
                              const float* hit_shade, float* grad_albedo, uint32_t mode, vx_stream stream);
