@@ -141,9 +141,12 @@ __device__ __forceinline__ void load_ray(const vx_ray* __restrict__ rays, uint64
 
 __device__ __forceinline__ void store_hit(vx_hit* __restrict__ out, uint64_t i, const vx_hit& h)
 {
-    uint4* dst = reinterpret_cast<uint4*>(out + i);
-    dst[0] = make_uint4(h.flags, __float_as_uint(h.t), h.leaf_node, h.brick);
-    dst[1] = make_uint4(h.voxel, __float_as_uint(h.rgb[0]), __float_as_uint(h.rgb[1]), __float_as_uint(h.rgb[2]));
+    // written once and never read back by the kernel: streamed (nt) stores
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4* dst = reinterpret_cast<u32x4*>(out + i);
+    __builtin_nontemporal_store(u32x4{h.flags, __float_as_uint(h.t), h.leaf_node, h.brick}, dst);
+    __builtin_nontemporal_store(u32x4{h.voxel, __float_as_uint(h.rgb[0]), __float_as_uint(h.rgb[1]),
+                                      __float_as_uint(h.rgb[2])}, dst + 1);
 }
 
 // BC1 decode of one texel (integer extraction, u32->f32 round-to-nearest, x 2^-32,
