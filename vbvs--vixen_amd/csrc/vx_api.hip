@@ -133,7 +133,13 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
         float o[3], d[3], tmin, tmax;
         load_ray(rays, i, o, d, tmin, tmax);
         vx_hit h;
-        trace_ray<COUNT, STAGED, true, LOD>(s, o, d, tmin, tmax, h, cnt);
+        // rays of one origin (a pinhole camera's) reach leaves together: visit them at once;
+        // otherwise postpone leaf visits until most of the wave waits at one
+        const bool same = f2u(o[0]) == __builtin_amdgcn_readfirstlane(f2u(o[0])) &&
+                          f2u(o[1]) == __builtin_amdgcn_readfirstlane(f2u(o[1])) &&
+                          f2u(o[2]) == __builtin_amdgcn_readfirstlane(f2u(o[2]));
+        if (__ballot(!same) == 0ull) trace_ray<COUNT, STAGED, false, LOD>(s, o, d, tmin, tmax, h, cnt);
+        else trace_ray<COUNT, STAGED, true, LOD>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
     if (COUNT) {
