@@ -3,8 +3,8 @@
 // The reference has no counterpart: VIXEN dispatches one scene per compute
 // pass (libraries/RenderGraph/src/Nodes/ComputeDispatchNode.cpp:249-317).
 // Batching changes dispatch and memory, never arithmetic: every ray runs the
-// same init_ray/step_ray code as vx_trace, so results are bit-identical to
-// per-scene launches.
+// same init_ray/esvo_step/visit_leaf code as vx_trace, so results are
+// bit-identical to per-scene launches.
 //
 // Memory: the scenes' descriptor, brick, occupancy and BC1 pools are copied
 // into four concatenated device pools; each scene's SceneView points at its
@@ -19,7 +19,11 @@
 // idle, hands them the next rays of its chunk (ranked by mbcnt), so divergent
 // rays do not leave lanes idle. A wave works on one scene at a time; when the
 // next chunk belongs to another scene it drains its lanes first, which keeps
-// the scene constants wave-uniform.
+// the scene constants wave-uniform. Leaf visits are postponed as in vx_trace's
+// incoherent waves: a lane that reaches a brick parks it until 3/4 of the
+// wave's live lanes wait, then they run their brick DDAs together (on the
+// 64-scene benchmark, half of it random-direction rays, 12% faster than
+// visiting at once).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,13 +81,14 @@ __global__ void __launch_bounds__(kMaxGroup)
     bool drained = false;
     SceneView sv = scenes[0].view; // every view carries the launch's per-lane LDS layout
 
-    bool active = false;
+    enum : int { IDLE = 0, ESVO = 1, LEAF = 2 };
+    int mode = IDLE;
     uint64_t ray_id = 0;
     RayState r;
     vx_hit h;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     for (;;) {
-        uint64_t idle = __ballot(!active);
+        uint64_t idle = __ballot(mode == IDLE);
         if ((uint32_t)__popcll(idle) >= refill_min || idle == full) {
             while (idle != 0ull) {
                 if (cur == end) {
@@ -118,7 +123,7 @@ __global__ void __launch_bounds__(kMaxGroup)
                 const uint64_t take = min((uint64_t)__popcll(idle), end - cur);
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                if (!active && rank < take) {
+                if (mode == IDLE && rank < take) {
                     ray_id = cur + rank;
                     float o[3], d[3], tmin, tmax;
                     load_ray(rays, ray_id, o, d, tmin, tmax);
@@ -126,21 +131,46 @@ __global__ void __launch_bounds__(kMaxGroup)
                         rec_load(sv, h);
                         store_hit(hits, ray_id, h);
                     } else {
-                        active = true;
+                        mode = ESVO;
                     }
                 }
                 cur += take;
-                idle = __ballot(!active);
+                idle = __ballot(mode == IDLE);
             }
         }
-        if (__ballot(active) == 0ull) {
+        const uint64_t live = __ballot(mode != IDLE);
+        if (live == 0ull) {
             if (drained && pending == kNoChunk && cur == end) break;
             continue;
         }
-        if (active && step_ray<false, STAGED>(sv, r, cnt)) {
+        // leaf visits postponed until 3/4 of the wave's live lanes wait at one
+        bool done = false;
+        if (mode == ESVO) {
+            LeafRef lf;
+            const int st = esvo_step<false, STAGED, false>(sv, r, cnt, lf);
+            if (st == STEP_DONE) {
+                done = true;
+            } else if (st == STEP_LEAF) {
+                *lane_word(sv, sv.aux_base, 11) = lf.brick;
+                *lane_word(sv, sv.aux_base, 12) = lf.lidx;
+                mode = LEAF;
+            }
+        }
+        const uint32_t waiting = (uint32_t)__popcll(__ballot(mode == LEAF));
+        if (waiting * 4u >= (uint32_t)__popcll(__ballot(mode != IDLE)) * 3u && mode == LEAF) {
+            int bc[3];
+            brick_cell(sv, r, bc);
+            if (visit_leaf<false>(sv, r, *lane_word(sv, sv.aux_base, 11), bc, *lane_word(sv, sv.aux_base, 12), cnt))
+                done = true;
+            else if (resume_after_leaf(sv, r))
+                done = true;
+            else
+                mode = ESVO;
+        }
+        if (done) {
             rec_load(sv, h);
             store_hit(hits, ray_id, h);
-            active = false;
+            mode = IDLE;
         }
     }
 }
@@ -196,7 +226,7 @@ static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tun
     // one per-lane layout for the launch: the deepest scene's stack
     uint32_t depth = 1;
     for (uint32_t s = 0; s < b->n; ++s) depth = std::max(depth, b->views[s].stack_depth);
-    const uint32_t lanes = t.waves_per_group * 64, lane_bytes = lanes * lane_lds_bytes(depth, false);
+    const uint32_t lanes = t.waves_per_group * 64, lane_bytes = lanes * lane_lds_bytes(depth, true);
     for (uint32_t s = 0; s < b->n; ++s) {
         if (off[s + 1] < off[s]) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: ray_offsets must be non-decreasing");
         BatchSceneInfo& e = tab[s];
@@ -214,7 +244,7 @@ static vx_status make_table(const vx_batch* b, const uint64_t* off, const vx_tun
     }
     for (BatchSceneInfo& e : tab) {
         e.view.stack_depth = depth;
-        lay_out_lanes(e.view, lds_acc, lanes, false); // step_ray: leaves are visited at once
+        lay_out_lanes(e.view, lds_acc, lanes, true); // leaf visits are postponed
     }
     n_chunks = chunk_acc;
     lds_bytes = lds_acc * 8u + lane_bytes;
