@@ -59,7 +59,9 @@ __device__ __forceinline__ uint32_t find_scene(const BatchSceneInfo* __restrict_
 __global__ void k_reset_queue(unsigned int* q) { *q = 0u; }
 
 template <bool STAGED>
-__global__ void __launch_bounds__(kMaxGroup)
+// 7 waves per SIMD (72 VGPRs, no scratch): 5% faster on the 64-scene batch than the 5 its
+// natural 85 VGPRs allow
+__global__ void __launch_bounds__(kMaxGroup) __attribute__((amdgpu_waves_per_eu(7, 7)))
     k_trace_persistent(const BatchSceneInfo* __restrict__ scenes, uint32_t n_scenes, uint64_t n_chunks,
                        uint32_t chunk, uint32_t refill_min, const vx_ray* __restrict__ rays,
                        vx_hit* __restrict__ hits, unsigned int* __restrict__ queue)
