@@ -15,7 +15,10 @@
  *     the text of the last failure on the calling thread is vx_last_error();
  *   - invalid rays (NaN/Inf, zero direction) produce a miss, never an error
  *     (libraries/SVO/src/SVOTraversal.cpp:238-259);
- *   - a context owns one device; scenes own their device copies;
+ *   - a context owns one device; scenes own their device copies; every call
+ *     on a scene, batch or graph runs on its context's device and leaves the
+ *     calling thread's current device as it found it (vx_camera_rays, which
+ *     takes no scene, runs on the current device);
  *   - trace/render calls are asynchronous on the stream they are given and
  *     take DEVICE pointers for rays, hits and images (inputs already resident
  *     in HBM); scene creation takes HOST pointers and copies.
@@ -221,7 +224,8 @@ vx_status vx_trace(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits
 vx_status vx_render(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
                     float tmax, vx_hit* hits, uint32_t* rgba8, vx_stream stream);
 
-/* The rays vx_render traces, written out (w*h vx_ray, device pointer). */
+/* The rays vx_render traces, written out (w*h vx_ray, device pointer; width
+ * and height at most 65536), on the calling thread's current device. */
 vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t height, float tmax,
                          vx_ray* rays, vx_stream stream);
 

@@ -140,6 +140,10 @@ def test_trace_rejects_null_arguments_without_device(vx):
     assert vx.LIB.vx_trace(None, None, 10, None, None) == 1
     assert vx.LIB.vx_render(None, None, 4, 4, 1.0, None, None, None) == 1
     assert vx.LIB.vx_camera_rays(None, 4, 4, 1.0, None, None) == 1
+    # oversized images are refused before anything reaches a device
+    cam = vx.VxCamera()
+    assert vx.LIB.vx_camera_rays(C.byref(cam), 70000, 4, 1.0, C.c_void_p(16), None) == 1
+    assert b"too large" in vx.LIB.vx_last_error()
     out = C.c_void_p()
     assert vx.LIB.vx_scene_create_device(None, None, C.byref(out)) == 1
 

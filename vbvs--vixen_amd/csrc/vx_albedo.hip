@@ -438,6 +438,7 @@ extern "C" {
 vx_status vx_scene_voxel_count(vx_scene* s, uint64_t* n_voxels)
 {
     if (!s || !n_voxels) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_voxel_count: null argument");
+    VX_DEVICE_GUARD(s->ctx->device);
     vx_status st;
     vx_albedo_state* a = vx::albedo_state(s, nullptr, st);
     if (!a) return st;
@@ -452,6 +453,7 @@ vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint3
         return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: bad arguments");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: image too large");
+    VX_DEVICE_GUARD(s->ctx->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     vx_status status;
     vx_albedo_state* a = vx::albedo_state(s, st, status);
@@ -474,6 +476,7 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
         return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: bad arguments");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: image too large");
+    VX_DEVICE_GUARD(s->ctx->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     vx_status status;
     vx_albedo_state* a = vx::albedo_state(s, st, status);

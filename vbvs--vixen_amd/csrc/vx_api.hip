@@ -527,9 +527,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
             return fail(VX_ERR_INVALID_SCENE, "node_count out of range");
         // the tree checks read the descriptors: one copy of them to the host
         std::vector<uint32_t> host(d->node_count * 2);
-        int dev = 0;
-        VX_HIP(hipGetDevice(&dev));
-        if (dev != ctx->device) VX_HIP(hipSetDevice(ctx->device));
+        VX_DEVICE_GUARD(ctx->device);
         if (hipMemcpy(host.data(), d->esvo_nodes, d->node_count * 8, hipMemcpyDeviceToHost) != hipSuccess)
             return fail(VX_ERR_INVALID_ARG, "vx_scene_create_device: esvo_nodes is not readable device memory");
         vx_scene_desc hd = *d;
@@ -540,9 +538,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     const uint32_t B = static_cast<uint32_t>(c.bricks_per_axis);
     const float sx = c.grid_max[0] - c.grid_min[0];
 
-    int cur = 0;
-    VX_HIP(hipGetDevice(&cur));
-    if (cur != ctx->device) VX_HIP(hipSetDevice(ctx->device));
+    VX_DEVICE_GUARD(ctx->device);
 
     auto* s = new (std::nothrow) vx_scene{};
     if (!s) return fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
@@ -647,6 +643,7 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
 {
     if (!s || (n && (!rays || !hits))) return fail(VX_ERR_INVALID_ARG, "vx_trace: null argument");
     if (n == 0) return VX_OK;
+    VX_DEVICE_GUARD(s->ctx->device);
     const uint64_t chunks = (n + kBlock - 1) / kBlock;
     const uint64_t per = (chunks + kXcds - 1) / kXcds;
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
@@ -741,6 +738,7 @@ vx_status vx_scene_validate(const vx_scene_desc* desc)
 vx_status vx_scene_destroy(vx_scene* s)
 {
     if (!s) return VX_OK;
+    const vx::DeviceGuard guard(s->ctx->device); // frees proceed even if it fails
     vx_albedo_state_free(s->albedo);
     if (!s->borrowed) {
         (void)hipFree(s->nodes);
@@ -818,6 +816,7 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     if (!s || !cam || (!hits && !rgba8)) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: null argument");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: image too large");
+    VX_DEVICE_GUARD(s->ctx->device);
     const vx::CamParams c = vx::make_cam(cam);
     const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
@@ -847,6 +846,8 @@ vx_status vx_bounce_rays(const vx_scene* s, const vx_ray* primary, const vx_hit*
 {
     if (!s || (n && (!primary || !hits || !out))) return vx::fail(VX_ERR_INVALID_ARG, "vx_bounce_rays: null argument");
     if (n == 0) return VX_OK;
+    if (n > 0xFFFFFFFFull * vx::kBlock) return vx::fail(VX_ERR_INVALID_ARG, "vx_bounce_rays: too many rays");
+    VX_DEVICE_GUARD(s->ctx->device);
     vx::k_bounce_rays<<<dim3((unsigned)((n + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0,
                         static_cast<hipStream_t>(stream)>>>(primary, hits, n, seed, s->view.k, out);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_bounce_rays: launch failed");
@@ -858,6 +859,7 @@ vx_status vx_camera_rays(const vx_camera* cam, uint32_t w, uint32_t h, float tma
     if (!cam || !rays) return vx::fail(VX_ERR_INVALID_ARG, "vx_camera_rays: null argument");
     const uint64_t n = (uint64_t)w * h;
     if (n == 0) return VX_OK;
+    if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_camera_rays: image too large");
     const vx::CamParams c = vx::make_cam(cam);
     vx::k_camera_rays<<<dim3((unsigned)((n + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0,
                         static_cast<hipStream_t>(stream)>>>(c, w, h, tmax, rays);

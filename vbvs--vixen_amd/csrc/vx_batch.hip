@@ -197,6 +197,7 @@ struct vx_batch {
 };
 
 struct vx_graph {
+    int device = 0; // the batch context's device
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     vx::BatchSceneInfo* d_table = nullptr;
@@ -292,6 +293,7 @@ static vx_status trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* 
 {
     if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null argument");
     if (off[b->n] > off[0] && (!rays || !hits)) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null rays/hits");
+    VX_DEVICE_GUARD(b->ctx->device);
     const vx_tuning t = sanitize(tuning, b);
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
@@ -312,6 +314,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
 {
     if (!ctx || !scenes || !out || n == 0) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: bad arguments");
     *out = nullptr;
+    VX_DEVICE_GUARD(ctx->device);
     uint64_t nodes = 0, bricks = 0, bc1_bricks = 0;
     for (uint32_t s = 0; s < n; ++s) {
         if (!scenes[s]) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: null scene");
@@ -377,6 +380,7 @@ static vx_status autotune(vx_batch* b, const vx_ray* rays, const uint64_t* off, 
                           float* best_ms, hipStream_t st)
 {
     if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_batch_autotune: null argument");
+    VX_DEVICE_GUARD(b->ctx->device);
     hipEvent_t e0, e1;
     VX_HIP(hipEventCreate(&e0));
     VX_HIP(hipEventCreate(&e1));
@@ -423,6 +427,7 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
 {
     if (!b || !off || !out) return fail(VX_ERR_INVALID_ARG, "vx_batch_graph_create: null argument");
     *out = nullptr;
+    VX_DEVICE_GUARD(b->ctx->device);
     const vx_tuning t = sanitize(tuning, b);
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
@@ -431,6 +436,7 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
     if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes, staged); s != VX_OK) return s;
     auto* g = new (std::nothrow) vx_graph{};
     if (!g) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: out of memory");
+    g->device = b->ctx->device;
     auto cleanup = [&](vx_status st) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -481,6 +487,7 @@ vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scene
 vx_status vx_batch_destroy(vx_batch* b)
 {
     if (!b) return VX_OK;
+    const vx::DeviceGuard guard(b->ctx->device); // frees proceed even if it fails
     (void)hipFree(b->nodes);
     (void)hipFree(b->brick_data);
     (void)hipFree(b->occ);
@@ -525,6 +532,7 @@ vx_status vx_batch_graph_create(vx_batch* b, const vx_ray* rays, const uint64_t*
 vx_status vx_graph_launch(vx_graph* g, vx_stream stream)
 {
     if (!g || !g->exec) return vx::fail(VX_ERR_INVALID_ARG, "vx_graph_launch: null graph");
+    VX_DEVICE_GUARD(g->device);
     if (hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) != hipSuccess)
         return vx::fail(VX_ERR_HIP, "vx_graph_launch: hipGraphLaunch failed");
     return VX_OK;
@@ -533,6 +541,7 @@ vx_status vx_graph_launch(vx_graph* g, vx_stream stream)
 vx_status vx_graph_destroy(vx_graph* g)
 {
     if (!g) return VX_OK;
+    const vx::DeviceGuard guard(g->device); // frees proceed even if it fails
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     (void)hipFree(g->d_table);

@@ -50,6 +50,40 @@ namespace vx {
             return ::vx::fail(VX_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
     } while (0)
 
+// Makes a context's device current for one API call and gives the caller's
+// current device back on return: null-stream launches, allocations and copies
+// then go to the device the scene lives on, whatever device the calling thread
+// had selected.
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int device)
+    {
+        int cur = -1;
+        err_ = hipGetDevice(&cur);
+        if (err_ == hipSuccess && cur != device) {
+            err_ = hipSetDevice(device);
+            if (err_ == hipSuccess) prev_ = cur;
+        }
+    }
+    ~DeviceGuard()
+    {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+    DeviceGuard(const DeviceGuard&) = delete;
+    DeviceGuard& operator=(const DeviceGuard&) = delete;
+    hipError_t error() const { return err_; }
+
+private:
+    int prev_ = -1;
+    hipError_t err_ = hipSuccess;
+};
+
+#define VX_DEVICE_GUARD(device)                                                                        \
+    ::vx::DeviceGuard vx_device_guard_(device);                                                       \
+    if (vx_device_guard_.error() != hipSuccess)                                                       \
+        return ::vx::fail(VX_ERR_HIP, std::string("could not make device ") + std::to_string(device) + \
+                                          " current: " + hipGetErrorString(vx_device_guard_.error()))
+
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kMaxLdsBytes = 65536;
 
