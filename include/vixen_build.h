@@ -46,6 +46,18 @@ typedef struct vx_built_scene {
  * N must be a power of two in [16, 4096]. nthreads <= 0 uses all cores. */
 vx_status vx_build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, int nthreads,
                               vx_built_scene** out);
+
+/* Sparse form of vx_build_from_dense, for worlds too large for a dense grid
+ * (up to 4096^3): the world as n populated bricks, as LaineKarrasOctree::rebuild
+ * collects them from the voxel world (libraries/SVO/src/SVORebuild.cpp:270-330:
+ * solid voxels binned by brick cell, bricks ordered by Morton code).
+ * Brick i has cell (brick_cells[3i], brick_cells[3i+1], brick_cells[3i+2]) in
+ * [0, N/8)^3 and RGBA8 voxels brick_voxels[512 i + z*64 + y*8 + x] (0 = empty).
+ * Bricks may come in any order; bricks with no solid voxel are dropped; two
+ * bricks in one cell are an error. The output equals vx_build_from_dense's on
+ * the dense grid holding the same voxels. */
+vx_status vx_build_from_bricks(const uint32_t* brick_cells, const uint32_t* brick_voxels, uint64_t n,
+                               uint32_t N, int with_bc1, int nthreads, vx_built_scene** out);
 void vx_built_scene_free(vx_built_scene* s);
 
 /* BC1 encode of 16 colours in [0,1]^3 (DXT1Compressor::encodeBlockTyped with

@@ -11,8 +11,10 @@
  * Parity status of each function (how it is pinned; see tests/test_oracle.py):
  *   orc_setup_ray     pinned by closed-form slab tests and the FP64 brute force
  *   trace_one         pinned by orc_bruteforce (hit voxel, brick, t) on every
- *                     non-tie ray of the Cornell and cave scenes, and by
- *                     hand-built trees with known answers
+ *                     non-tie ray of the Cornell and cave scenes, by an FP64
+ *                     slab test against every solid voxel of sparse 2048^3 and
+ *                     4096^3 worlds (the deepest trees), and by hand-built
+ *                     trees with known answers
  *   orc_bc1_decode    pinned exhaustively: all 4 selectors x random heads
  *                     against the integer RGB565 closed form
  *   orc_camera_rays   pinned by closed-form pixel-centre directions

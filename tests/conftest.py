@@ -87,3 +87,43 @@ def leaving_rays(ray_dtype, N, n_per, seed):
         rays["o"][sl], rays["d"][sl] = o, d
     rays["tmax"] = 3.0e38
     return rays
+
+
+def sparse_world(N, seed, n_scattered=60, cluster=4, density=0.3):
+    """A sparse world for vx_build_from_bricks: scattered bricks (two in opposite
+    corner cells), a cluster of cluster^3 adjacent bricks, seeded RGBA8 voxels
+    at the given density. Returns cells [n, 3] (x, y, z), voxels [n, 8, 8, 8]
+    (z, y, x) and the global (x, y, z) of every solid voxel."""
+    rng = np.random.default_rng(seed)
+    B = N // 8
+    cells = {(0, 0, 0), (B - 1, B - 1, B - 1)}
+    while len(cells) < n_scattered:
+        cells.add(tuple(int(v) for v in rng.integers(0, B, 3)))
+    c0 = rng.integers(0, B - cluster, 3)
+    for dz in range(cluster):
+        for dy in range(cluster):
+            for dx in range(cluster):
+                cells.add((int(c0[0] + dx), int(c0[1] + dy), int(c0[2] + dz)))
+    cells = np.array(sorted(cells), np.uint32)
+    n = len(cells)
+    solid = rng.random((n, 8, 8, 8)) < density
+    solid[:, 0, 0, 0] = True  # no brick is empty
+    vox = np.where(solid, rng.integers(0, 2 ** 24, (n, 8, 8, 8)).astype(np.uint32) | np.uint32(0xFF000000), 0)
+    vox = vox.astype(np.uint32)
+    b, z, y, x = np.nonzero(vox)
+    solid_xyz = np.stack([cells[b, 0] * 8 + x, cells[b, 1] * 8 + y, cells[b, 2] * 8 + z], 1).astype(np.int64)
+    perm = rng.permutation(n)
+    return cells[perm], vox[perm], solid_xyz
+
+
+def aimed_rays(ray_dtype, rng, n, N, solid_xyz):
+    """Rays from random origins (half inside the volume, half around it) towards
+    random points of random solid voxels."""
+    r = np.zeros(n, ray_dtype)
+    k = n // 2
+    r["o"][:k] = rng.uniform(0, N, (k, 3))
+    r["o"][k:] = rng.uniform(-0.2 * N, 1.2 * N, (n - k, 3))
+    tgt = solid_xyz[rng.integers(0, len(solid_xyz), n)] + rng.uniform(0.05, 0.95, (n, 3))
+    r["d"] = tgt - r["o"]
+    r["tmax"] = 3.0e38
+    return r

@@ -216,3 +216,72 @@ def test_invalid_sizes_raise(vx):
         vx.build_from_dense(np.zeros((24, 24, 24), np.uint32))
     with pytest.raises(vx.VixenError):
         vx.gen_cave(100, 1, 0.5)
+
+
+# ----------------------------------------------------------------------------
+# Sparse builder (vx_build_from_bricks)
+# ----------------------------------------------------------------------------
+def _bricks_of(grid):
+    """Every brick cell of a dense grid[z, y, x] (empty ones included): cells (x, y, z), voxels [n, z, y, x]."""
+    N = grid.shape[0]
+    B = N // 8
+    vox = grid.reshape(B, 8, B, 8, B, 8).transpose(0, 2, 4, 1, 3, 5).reshape(-1, 8, 8, 8)  # [bz, by, bx]
+    bz, by, bx = np.meshgrid(np.arange(B), np.arange(B), np.arange(B), indexing="ij")
+    cells = np.stack([bx.ravel(), by.ravel(), bz.ravel()], 1).astype(np.uint32)
+    return cells, vox
+
+
+def _same_built(a, b):
+    assert np.array_equal(a.nodes, b.nodes)
+    assert np.array_equal(a.brick_data, b.brick_data)
+    assert (a.bc1 is None) == (b.bc1 is None) and (a.bc1 is None or np.array_equal(a.bc1, b.bc1))
+    assert np.array_equal(a.brick_coords, b.brick_coords)
+    assert bytes(a.config) == bytes(b.config) and a.internal_count == b.internal_count
+
+
+@pytest.mark.parametrize("kind,N,seed", [("cave", 64, 3), ("urban", 128, 4), ("random", 32, 5), ("empty", 16, 0)])
+def test_sparse_build_equals_dense(vx, kind, N, seed):
+    if kind == "cave":
+        grid, _ = vx.gen_cave(N, seed, 0.4)
+    elif kind == "urban":
+        grid, _ = vx.gen_urban(N, seed, 0.7)
+    elif kind == "random":
+        grid = _random_scene(N, 0.01, seed)
+    else:
+        grid = np.zeros((N, N, N), np.uint32)
+    cells, vox = _bricks_of(grid)
+    perm = np.random.default_rng(seed).permutation(len(cells))  # any order; empty bricks are dropped
+    for bc1 in (True, False):
+        _same_built(vx.build_from_bricks(cells[perm], vox[perm], N, bc1=bc1), vx.build_from_dense(grid, bc1=bc1))
+
+
+def test_sparse_build_rejects_bad_input(vx):
+    cells = np.array([[0, 0, 0], [1, 0, 0]], np.uint32)
+    vox = np.full((2, 8, 8, 8), 0xFF0000FF, np.uint32)
+    vx.build_from_bricks(cells, vox, 16)
+    with pytest.raises(vx.VixenError, match="share a cell"):
+        vx.build_from_bricks(np.array([[1, 0, 0], [1, 0, 0]], np.uint32), vox, 16)
+    with pytest.raises(vx.VixenError, match="outside"):
+        vx.build_from_bricks(np.array([[0, 0, 0], [2, 0, 0]], np.uint32), vox, 16)
+    with pytest.raises(vx.VixenError):
+        vx.build_from_bricks(cells, vox, 24)
+    # a duplicate cell whose brick is empty is dropped, not an error
+    dup = vx.build_from_bricks(np.array([[0, 0, 0], [1, 0, 0], [1, 0, 0]], np.uint32),
+                               np.concatenate([vox, np.zeros((1, 8, 8, 8), np.uint32)]), 16)
+    _same_built(dup, vx.build_from_bricks(cells, vox, 16))
+
+
+@pytest.mark.parametrize("N", [2048, 4096])
+def test_sparse_build_at_maximum_size(vx, N):
+    """The largest grids the traversal stack supports: 8 and 9 levels above the bricks."""
+    from conftest import sparse_world
+    cells, vox, _ = sparse_world(N, 21, n_scattered=40, cluster=3)
+    built = vx.build_from_bricks(cells, vox, N)
+    B = N // 8
+    assert built.config.bricks_per_axis == B and built.config.user_max_levels == B.bit_length() - 1 + 3
+    assert built.brick_count == len(cells)
+    # brick ids are Morton ranks of the cells, and every descriptor is in range
+    codes = [_morton(int(x), int(y), int(z)) for x, y, z in built.brick_coords]
+    assert codes == sorted(codes)
+    from test_abi import _validate
+    assert _validate(vx, built)[0] == 0

@@ -95,3 +95,34 @@ def test_render_tiny_images_bit_exact(vx, orc, edge_scenes, gpu_ctx, name):
             img = rgba.cpu().numpy().view(np.uint32)
             want_img = orc.render(oscene, cam.as_floats(), w, h, 1e30, nthreads=4)
             assert np.array_equal(img, want_img), f"{name} camera {ci} {w}x{h}: RGBA8 differs"
+
+
+@pytest.mark.parametrize("N", [2048, 4096])
+def test_maximum_size_sparse_world_bit_exact(vx, orc, gpu_ctx, N):
+    """The deepest trees the traversal stack supports (8 and 9 levels above the
+    bricks), built from a brick list: aimed and random rays, a camera frame, and
+    the same with the top descriptor levels staged in LDS."""
+    import torch
+    from conftest import aimed_rays, sparse_world
+    cells, voxels, solid = sparse_world(N, 21, n_scattered=40, cluster=3)
+    built = vx.build_from_bricks(cells, voxels, N, bc1=True)
+    scene = vx.Scene(gpu_ctx, built)
+    oscene = orc.OracleScene.from_built(built)
+    rng = np.random.default_rng(N + 1)
+    rays = np.concatenate([aimed_rays(orc.RAY_DTYPE, rng, 20_000, N, solid), _rays(orc, rng, 20_000, N)])
+    want, _ = orc.trace(oscene, rays, nthreads=8)
+    assert (want["flags"] & 1).mean() > 0.4
+    target = solid[len(solid) // 2] + 0.5
+    eye = target + np.array([-30.0, 25.0, -35.0])
+    cam = vx.make_camera(eye, target - eye, 60.0, 2.0)
+    crays = orc.camera_rays(cam.as_floats(), 258, 129, 1e30)
+    cwant, _ = orc.trace(oscene, crays, nthreads=8)
+    assert (cwant["flags"] & 1).mean() > 0.05
+    for depth in (0, 3):
+        scene.set_lds_depth(depth)
+        got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+        torch.cuda.synchronize()
+        _check(got, want, f"N={N} rays, lds depth {depth}")
+        hits, _ = scene.render(cam, 258, 129, tmax=1e30)
+        torch.cuda.synchronize()
+        _check(vx.hits_to_numpy(hits), cwant, f"N={N} camera, lds depth {depth}")

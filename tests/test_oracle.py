@@ -284,3 +284,79 @@ def test_tmin_skips_near_voxels(vx, orc):
     assert not (h["flags"][2] & 1)               # the segment ends before the voxel
     assert h["flags"][3] & 1 and h["t"][3] == 2.0  # ...or just inside it
     assert h["flags"][4] & 1 and h["t"][4] == 2.5  # starting inside a solid voxel: t clamps to tmin
+
+
+# ----------------------------------------------------------------------------
+# Maximum-size sparse worlds against an FP64 slab test per solid voxel
+# ----------------------------------------------------------------------------
+def _sparse_bruteforce(orc, scene, solid_xyz, rays, eps=2e-5):
+    """First solid voxel along each ray by an FP64 slab test against every solid
+    voxel (for worlds too large for a dense grid). The segment is the oracle's
+    own set-up [ts, te] in voxel units; ties within eps (shared faces, edges,
+    corners, a start on a voxel face, a hit at the segment's end) are ambiguous."""
+    e = lambda x: eps * max(1.0, abs(x))  # noqa: E731
+    V = solid_xyz.astype(np.float64)
+    n = len(rays)
+    vox = np.full((n, 3), -1, np.int64)
+    t = np.full(n, np.inf)
+    amb = np.zeros(n, bool)
+    for i in range(n):
+        su = orc.setup_ray(scene, rays[i])
+        if su is None:
+            continue
+        ol, dl, _, ts, te = su
+        o = ol.astype(np.float64)
+        d = np.where(np.abs(dl) < 1e-8, 0.0, dl.astype(np.float64))
+        lo = np.full(len(V), -np.inf)
+        hi = np.full(len(V), np.inf)
+        for a in range(3):
+            if d[a] != 0.0:
+                t0 = (V[:, a] - o[a]) / d[a]
+                t1 = (V[:, a] + 1.0 - o[a]) / d[a]
+                lo = np.maximum(lo, np.minimum(t0, t1))
+                hi = np.minimum(hi, np.maximum(t0, t1))
+            else:
+                inside = (V[:, a] <= o[a]) & (o[a] < V[:, a] + 1.0)
+                lo = np.where(inside, lo, np.inf)
+                hi = np.where(inside, hi, -np.inf)
+        entry = np.maximum(lo, ts)
+        ok = (entry <= hi) & (hi > ts) & (entry <= te)
+        soft = (entry <= hi + eps * np.maximum(1.0, np.abs(hi))) & (hi >= ts - e(ts)) & (entry <= te + e(te))
+        if not ok.any():
+            amb[i] = soft.any()
+            continue
+        cand = np.nonzero(ok)[0]
+        b = cand[np.argmin(entry[cand])]
+        tb = entry[b]
+        others = soft.copy()
+        others[b] = False
+        amb[i] = bool((entry[others] <= tb + e(tb)).any()) or abs(tb - te) <= e(te)
+        vox[i] = solid_xyz[b]
+        t[i] = tb
+    return vox, t, amb
+
+
+@pytest.mark.parametrize("N", [2048, 4096])
+def test_oracle_vs_sparse_bruteforce_maximum_size(vx, orc, N):
+    from conftest import aimed_rays, sparse_world
+    cells, voxels, solid = sparse_world(N, 21, n_scattered=40, cluster=3)
+    built = vx.build_from_bricks(cells, voxels, N, bc1=False)
+    scene = orc.OracleScene.from_built(built)
+    rng = np.random.default_rng(N)
+    rays = aimed_rays(orc.RAY_DTYPE, rng, 1500, N, solid)
+    extra = np.zeros(500, orc.RAY_DTYPE)
+    extra["o"] = rng.uniform(-0.2 * N, 1.2 * N, (500, 3))
+    extra["d"] = rng.normal(size=(500, 3))
+    extra["tmax"] = 3.0e38
+    rays = np.concatenate([rays, extra])
+    hits, _ = orc.trace(scene, rays, nthreads=8)
+    vox, t, amb = _sparse_bruteforce(orc, scene, solid, rays, eps=4e-6)
+    got = global_voxels(built, hits)
+    ok = (got == vox).all(1) | amb
+    bad = np.nonzero(~ok)[0]
+    assert len(bad) == 0, f"{len(bad)} rays disagree; first {rays[bad[0]]}: oracle {got[bad[0]]} brute force {vox[bad[0]]}"
+    assert amb.mean() < 0.01
+    hit = (vox[:, 0] >= 0) & ~amb
+    assert hit.mean() > 0.5  # the aimed rays mostly hit
+    rel = np.abs(hits["t"][hit].astype(np.float64) - t[hit]) / np.maximum(1.0, np.abs(t[hit]))
+    assert rel.max() <= 1e-5

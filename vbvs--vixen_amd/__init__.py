@@ -157,6 +157,7 @@ def _load() -> C.CDLL:
         "vx_camera_rays": ([C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp], st),
         "vx_bounce_rays": ([vp, vp, vp, u64, u32, vp, vp], st),
         "vx_build_from_dense": ([vp, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
+        "vx_build_from_bricks": ([vp, vp, u64, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
         "vx_encode_normals": ([vp, u64, C.c_int, vp], st),
@@ -194,7 +195,7 @@ LIB = _load()
 # Every symbol the headers declare (the CPU suite checks the library exports them all).
 EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create", "vx_scene_validate",
             "vx_scene_destroy", "vx_scene_info", "vx_trace", "vx_trace_counters", "vx_render",
-            "vx_camera_rays", "vx_build_from_dense", "vx_built_scene_free", "vx_bc1_encode_block",
+            "vx_camera_rays", "vx_build_from_dense", "vx_build_from_bricks", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",
             "vx_batch_destroy", "vx_trace_batched", "vx_batch_autotune", "vx_batch_graph_create", "vx_graph_launch",
             "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
@@ -304,6 +305,24 @@ def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0) -> B
     assert grid.shape == (N, N, N)
     out = C.POINTER(VxBuiltScene)()
     _check(LIB.vx_build_from_dense(_ptr(grid), N, int(bc1), nthreads, C.byref(out)), "vx_build_from_dense")
+    return _take_built(out, bc1)
+
+
+def build_from_bricks(cells: np.ndarray, voxels: np.ndarray, N: int, bc1: bool = True,
+                      nthreads: int = 0) -> BuiltScene:
+    """Sparse world -> octree buffers: cells [n, 3] brick cells (x, y, z) in [0, N/8),
+    voxels [n, 8, 8, 8] RGBA8 as voxels[i, z, y, x] (0 = empty)."""
+    cells = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1, 3)
+    voxels = np.ascontiguousarray(voxels, dtype=np.uint32).reshape(-1, 512)
+    assert len(cells) == len(voxels)
+    out = C.POINTER(VxBuiltScene)()
+    _check(LIB.vx_build_from_bricks(_ptr(cells), _ptr(voxels), len(cells), N, int(bc1), nthreads, C.byref(out)),
+           "vx_build_from_bricks")
+    return _take_built(out, bc1)
+
+
+def _take_built(out, bc1: bool) -> BuiltScene:
+    """Copies a vx_built_scene into numpy arrays and frees it."""
     try:
         s = out.contents
         nc, nb = int(s.node_count), int(s.brick_count)

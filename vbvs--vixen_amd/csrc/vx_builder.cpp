@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace vx {
@@ -165,46 +167,22 @@ static void free_built(vx_built_scene* s)
     std::free(s);
 }
 
-static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, int nthreads,
-                                  vx_built_scene** out)
+// The buffers of a scene whose populated brick cells have the Morton codes
+// `codes` (ascending, unique): fill(id, code, dst) writes the 512 voxels of
+// brick `id` (cell `code`) to dst, z-major (dst[z*64 + y*8 + x]). `who` names
+// the entry point in error messages.
+template <class Fill>
+static vx_status build_tree(const std::vector<uint64_t>& codes, uint32_t N, int with_bc1, int nthreads,
+                            Fill&& fill, vx_built_scene** out, const char* who)
 {
-    if (!grid || !out) return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: null argument");
-    if (N < 16 || N > 4096 || (N & (N - 1)) != 0)
-        return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: N must be a power of two in [16, 4096]");
-    *out = nullptr;
     const uint32_t B = N / 8;
     int levels = 0; // log2(B): internal levels above the bricks (root included)
     while ((1u << levels) < B) ++levels;
-    const uint64_t cells = static_cast<uint64_t>(B) * B * B;
-
-    // 1. Occupancy of every brick cell, visited in Morton order.
-    std::vector<uint8_t> occ(cells, 0);
-    parallel_for(cells, nthreads, [&](uint64_t b, uint64_t e) {
-        for (uint64_t m = b; m < e; ++m) {
-            const uint32_t bx = compact3(m), by = compact3(m >> 1), bz = compact3(m >> 2);
-            bool any = false;
-            for (uint32_t z = 0; z < 8 && !any; ++z)
-                for (uint32_t y = 0; y < 8 && !any; ++y) {
-                    const uint32_t* row = grid + (static_cast<uint64_t>(bz * 8 + z) * N + (by * 8 + y)) * N + bx * 8;
-                    for (uint32_t x = 0; x < 8; ++x)
-                        if (row[x] != 0u) {
-                            any = true;
-                            break;
-                        }
-                }
-            occ[m] = any ? 1 : 0;
-        }
-    });
-
-    // 2. Brick ids = Morton ranks of the populated cells.
-    std::vector<uint64_t> codes;
-    for (uint64_t m = 0; m < cells; ++m)
-        if (occ[m]) codes.push_back(m);
     const uint64_t nb = codes.size();
-    if (nb >= 0xFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: more bricks than 24-bit ids");
+    if (nb >= 0xFFFFFFull) return fail(VX_ERR_INVALID_ARG, std::string(who) + ": more bricks than 24-bit ids");
 
-    // 3. Internal levels (bottom-up): level l holds the unique codes >> 3l.
-    //    lvl[0] = bricks, lvl[levels] = {0} (root).
+    // Internal levels (bottom-up): level l holds the unique codes >> 3l.
+    // lvl[0] = bricks, lvl[levels] = {0} (root).
     std::vector<std::vector<uint64_t>> lvl(levels + 1);
     lvl[0] = codes;
     for (int l = 1; l <= levels; ++l) {
@@ -224,10 +202,10 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
         total += lvl[l].size();
     }
     if (total >= (1ull << 24))
-        return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: descriptor count exceeds 24-bit far pointers");
+        return fail(VX_ERR_INVALID_ARG, std::string(who) + ": descriptor count exceeds 24-bit far pointers");
 
     auto* s = static_cast<vx_built_scene*>(std::calloc(1, sizeof(vx_built_scene)));
-    if (!s) return fail(VX_ERR_OUT_OF_MEMORY, "vx_build_from_dense: out of memory");
+    if (!s) return fail(VX_ERR_OUT_OF_MEMORY, std::string(who) + ": out of memory");
     s->node_count = total;
     s->brick_count = nb;
     s->internal_count = total - nb;
@@ -237,11 +215,11 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
     if (with_bc1) s->bc1_colors = static_cast<uint64_t*>(std::malloc(std::max<uint64_t>(1, nb * 32) * sizeof(uint64_t)));
     if (!s->esvo_nodes || !s->brick_data || !s->brick_coords || (with_bc1 && !s->bc1_colors)) {
         free_built(s);
-        return fail(VX_ERR_OUT_OF_MEMORY, "vx_build_from_dense: out of memory");
+        return fail(VX_ERR_OUT_OF_MEMORY, std::string(who) + ": out of memory");
     }
 
-    // 4. Internal descriptors: children of a node are contiguous in the next
-    //    level down (sorted codes), in octant order.
+    // Internal descriptors: children of a node are contiguous in the next
+    // level down (sorted codes), in octant order.
     for (int l = levels; l >= 1; --l) {
         const auto& cur = lvl[l];
         const auto& child = lvl[l - 1];
@@ -269,7 +247,7 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
         }
     }
 
-    // 5. Leaf descriptors, brick pool and BC1 blocks.
+    // Leaf descriptors, brick pool and BC1 blocks.
     parallel_for(nb, nthreads, [&](uint64_t b, uint64_t e) {
         for (uint64_t id = b; id < e; ++id) {
             const uint64_t m = codes[id];
@@ -278,11 +256,7 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
             s->esvo_nodes[2 * (offset[0] + id) + 1] = static_cast<uint32_t>(id) & 0xFFFFFFu;
             s->brick_coords[id] = bx | (by << 10) | (bz << 20);
             uint32_t* dst = s->brick_data + id * 512;
-            for (uint32_t z = 0; z < 8; ++z)
-                for (uint32_t y = 0; y < 8; ++y) {
-                    const uint32_t* row = grid + (static_cast<uint64_t>(bz * 8 + z) * N + (by * 8 + y)) * N + bx * 8;
-                    std::memcpy(dst + z * 64 + y * 8, row, 8 * sizeof(uint32_t));
-                }
+            fill(id, m, dst);
             if (with_bc1) {
                 for (int blk = 0; blk < 32; ++blk) {
                     const uint32_t* w = dst + blk * 16;
@@ -302,7 +276,7 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
         }
     });
 
-    // 6. OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it (voxel units).
+    // OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it (voxel units).
     vx_octree_config& c = s->config;
     std::memset(&c, 0, sizeof c);
     const int L = levels + 3;
@@ -328,6 +302,84 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
     return VX_OK;
 }
 
+static bool valid_edge(uint32_t N) { return N >= 16 && N <= 4096 && (N & (N - 1)) == 0; }
+
+static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, int nthreads,
+                                  vx_built_scene** out)
+{
+    if (!grid || !out) return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: null argument");
+    if (!valid_edge(N))
+        return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: N must be a power of two in [16, 4096]");
+    *out = nullptr;
+    const uint32_t B = N / 8;
+    const uint64_t cells = static_cast<uint64_t>(B) * B * B;
+
+    // 1. Occupancy of every brick cell, visited in Morton order.
+    std::vector<uint8_t> occ(cells, 0);
+    parallel_for(cells, nthreads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t m = b; m < e; ++m) {
+            const uint32_t bx = compact3(m), by = compact3(m >> 1), bz = compact3(m >> 2);
+            bool any = false;
+            for (uint32_t z = 0; z < 8 && !any; ++z)
+                for (uint32_t y = 0; y < 8 && !any; ++y) {
+                    const uint32_t* row = grid + (static_cast<uint64_t>(bz * 8 + z) * N + (by * 8 + y)) * N + bx * 8;
+                    for (uint32_t x = 0; x < 8; ++x)
+                        if (row[x] != 0u) {
+                            any = true;
+                            break;
+                        }
+                }
+            occ[m] = any ? 1 : 0;
+        }
+    });
+
+    // 2. Brick ids = Morton ranks of the populated cells.
+    std::vector<uint64_t> codes;
+    for (uint64_t m = 0; m < cells; ++m)
+        if (occ[m]) codes.push_back(m);
+    auto fill = [&](uint64_t, uint64_t m, uint32_t* dst) {
+        const uint32_t bx = compact3(m), by = compact3(m >> 1), bz = compact3(m >> 2);
+        for (uint32_t z = 0; z < 8; ++z)
+            for (uint32_t y = 0; y < 8; ++y) {
+                const uint32_t* row = grid + (static_cast<uint64_t>(bz * 8 + z) * N + (by * 8 + y)) * N + bx * 8;
+                std::memcpy(dst + z * 64 + y * 8, row, 8 * sizeof(uint32_t));
+            }
+    };
+    return build_tree(codes, N, with_bc1, nthreads, fill, out, "vx_build_from_dense");
+}
+
+static vx_status build_from_bricks(const uint32_t* cells, const uint32_t* voxels, uint64_t n, uint32_t N,
+                                   int with_bc1, int nthreads, vx_built_scene** out)
+{
+    if (!out || (n && (!cells || !voxels))) return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: null argument");
+    if (!valid_edge(N))
+        return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: N must be a power of two in [16, 4096]");
+    *out = nullptr;
+    const uint32_t B = N / 8;
+    // (Morton code, input index) of every brick with a solid voxel
+    std::vector<std::pair<uint64_t, uint64_t>> order;
+    order.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t x = cells[3 * i], y = cells[3 * i + 1], z = cells[3 * i + 2];
+        if (x >= B || y >= B || z >= B)
+            return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: brick cell " + std::to_string(i) +
+                                                " outside the " + std::to_string(B) + "^3 brick grid");
+        const uint32_t* v = voxels + i * 512;
+        if (std::any_of(v, v + 512, [](uint32_t w) { return w != 0u; })) order.emplace_back(morton3(x, y, z), i);
+    }
+    std::sort(order.begin(), order.end());
+    for (size_t k = 1; k < order.size(); ++k)
+        if (order[k].first == order[k - 1].first)
+            return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: bricks " + std::to_string(order[k - 1].second) +
+                                                " and " + std::to_string(order[k].second) + " share a cell");
+    std::vector<uint64_t> codes(order.size());
+    for (size_t k = 0; k < order.size(); ++k) codes[k] = order[k].first;
+    auto fill = [&](uint64_t id, uint64_t, uint32_t* dst) {
+        std::memcpy(dst, voxels + order[id].second * 512, 512 * sizeof(uint32_t));
+    };
+    return build_tree(codes, N, with_bc1, nthreads, fill, out, "vx_build_from_bricks");
+}
+
 } // namespace vx
 
 extern "C" {
@@ -340,6 +392,18 @@ vx_status vx_build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, in
         return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_build_from_dense: out of memory");
     } catch (...) {
         return vx::fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: unexpected exception");
+    }
+}
+
+vx_status vx_build_from_bricks(const uint32_t* brick_cells, const uint32_t* brick_voxels, uint64_t n, uint32_t N,
+                               int with_bc1, int nthreads, vx_built_scene** out)
+{
+    try {
+        return vx::build_from_bricks(brick_cells, brick_voxels, n, N, with_bc1, nthreads, out);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_build_from_bricks: out of memory");
+    } catch (...) {
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: unexpected exception");
     }
 }
 
