@@ -177,3 +177,23 @@ def test_random_stress(make):
         else:
             misses += 1
     assert hits > 0 and misses > 0
+
+
+def test_structure_queries(vx, cave):
+    """getWorldMin/Max, getMaxLevels, getVoxelSize and getGPUBuffers of the castRay adapter
+    (LaineKarrasOctree.h:97-108, LaineKarrasOctree.cpp:327-334), on a host-built scene."""
+    grid, cam, built = cave
+
+    class _HostScene:  # the adapter reads only the config and the host buffers
+        def __init__(self, b):
+            self.built, self._cfg = b, b.config
+
+    octree = vx.LaineKarrasOctreeHip(_HostScene(built))
+    assert octree.getWorldMin() == (0.0, 0.0, 0.0) and octree.getWorldMax() == (256.0, 256.0, 256.0)
+    assert octree.getMaxLevels() == 8  # 5 levels above 8^3 bricks
+    assert [octree.getVoxelSize(s) for s in (0, 1, 7, 8, 9)] == [256.0, 128.0, 2.0, 0.0, 0.0]
+    bufs = octree.getGPUBuffers()
+    assert len(bufs["hierarchyBuffer"]) == 8 * built.node_count
+    assert len(bufs["attributeBuffer"]) == 2048 * built.brick_count
+    assert len(bufs["compressedColorBuffer"]) == 256 * built.brick_count  # test_block_compressor.cpp:248-260
+    assert bufs["compressedNormalBuffer"] == b""

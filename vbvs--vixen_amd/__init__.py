@@ -797,6 +797,36 @@ class LaineKarrasOctreeHip:
     def __init__(self, scene: Scene):
         self.scene = scene
 
+    # ---- the structure queries a castRay caller uses (libraries/SVO/include/LaineKarrasOctree.h:97-108) ----
+    def getWorldMin(self) -> tuple[float, float, float]:
+        return tuple(float(x) for x in self.scene._cfg.grid_min)
+
+    def getWorldMax(self) -> tuple[float, float, float]:
+        return tuple(float(x) for x in self.scene._cfg.grid_max)
+
+    def getMaxLevels(self) -> int:
+        return int(self.scene._cfg.user_max_levels)
+
+    def getVoxelSize(self, scale: int) -> float:
+        """LaineKarrasOctree::getVoxelSize (libraries/SVO/src/LaineKarrasOctree.cpp:327-334): the world's x
+        extent over 2^scale in float32, 0 from scale = getMaxLevels() on."""
+        if scale >= self.getMaxLevels():
+            return 0.0
+        ext = np.float32(self.scene._cfg.grid_max[0]) - np.float32(self.scene._cfg.grid_min[0])
+        return float(ext / np.float32(2.0 ** scale))
+
+    def getGPUBuffers(self) -> dict:
+        """ISVOStructure::GPUBuffers (libraries/SVO/include/ISVOStructure.h:252-269) of a host-built scene:
+        hierarchy = the child descriptors, attribute = brickData, and the DXT colour and normal pools (empty
+        when absent)."""
+        b = self.scene.built
+        if b is None:
+            raise ValueError("getGPUBuffers: the scene was created over device buffers")
+        none = b""
+        return {"hierarchyBuffer": b.nodes.tobytes(), "attributeBuffer": b.brick_data.tobytes(), "auxBuffer": none,
+                "compressedColorBuffer": b.bc1.tobytes() if b.bc1 is not None else none,
+                "compressedNormalBuffer": b.normals.tobytes() if b.normals is not None else none}
+
     def castRays(self, rays: np.ndarray, lod: VxLod | None = None) -> np.ndarray:
         d = rays_to_device(np.asarray(rays, dtype=RAY_DTYPE), self.scene.ctx.device)
         out = hits_to_numpy(self.scene.trace(d) if lod is None else self.scene.trace_lod(d, lod))
