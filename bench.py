@@ -37,6 +37,12 @@ Also:
                 200k rays of the frame and the CPU model: reported, not a target
   parity        the last headline frame compared bit for bit with the oracle's
 
+--urban adds one more leg (off by default: it builds a 2 GB scene):
+  urban     the 1024^3 urban box grid (BC1, far pointers, 7 levels above the
+            bricks) at 2560x1440: vx_render of the primary rays, and vx_trace of
+            one seeded cosine bounce ray per primary hit; both ray sets compared
+            with the oracle's unless --no-cpu.
+
 --vixen-json PATH also writes the headline frames in the per-test results schema
 of VIXEN's benchmark runner (libraries/Profiler/src/MetricsExporter.cpp:117-230),
 pipeline "hip", so its results tooling can chart this backend beside the Vulkan
@@ -289,6 +295,45 @@ def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int):
             "bounce_mrays": round(W * H / ms_bounce / 1e3, 1)}
 
 
+def urban_leg(vx, orc, torch, ctx, stream, steps: int, check: bool):
+    W3, H3 = 2560, 1440
+    t0 = time.perf_counter()
+    grid, cam = vx.gen_urban(1024, 3, 1.0)
+    built = vx.build_from_dense(grid, bc1=True)
+    del grid
+    build_s = time.perf_counter() - t0
+    scene = vx.Scene(ctx, built)
+    dev = torch.device("cuda", ctx.device)
+    hits = torch.empty(W3 * H3 * 32, dtype=torch.uint8, device=dev)
+
+    def render():
+        vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(cam), W3, H3, 3.0e38, hits.data_ptr(), None,
+                                   stream.cuda_stream), "vx_render")
+
+    ms_primary = timed(torch, stream, render, steps, 2)
+    primary = vx.camera_rays(cam, W3, H3, 3.0e38, stream=stream)
+    bounce = scene.bounce_rays(primary, hits, seed=1234, stream=stream)
+    h_bounce = torch.empty_like(bounce)
+    ms_bounce = timed(torch, stream, lambda: scene.trace(bounce, h_bounce, stream=stream), steps, 2)
+    torch.cuda.synchronize()
+    n = W3 * H3
+    out = {"workload": "urban1024_bc1_1440p: 1024^3 urban box grid (seed 3), 2560x1440 primary rays and one "
+                       "cosine bounce ray per primary hit (seed 1234)",
+           "bricks": built.brick_count, "descriptors": built.node_count, "host_build_s": round(build_s, 2),
+           "primary_ms": round(ms_primary, 4), "primary_mrays": round(n / ms_primary / 1e3, 1),
+           "bounce_ms": round(ms_bounce, 4), "bounce_rays": n, "bounce_mrays": round(n / ms_bounce / 1e3, 1)}
+    if check:
+        oscene = orc.OracleScene.from_built(built)
+        threads = min(16, os.cpu_count() or 1)
+        ok = True
+        for rays_dev, got_dev in ((primary, hits), (bounce, h_bounce)):
+            want, _ = orc.trace(oscene, rays_dev.cpu().numpy().view(orc.RAY_DTYPE), nthreads=threads)
+            ok &= vx.hits_to_numpy(got_dev).tobytes() == want.tobytes()
+        out["parity"] = "bit-exact vs oracle" if ok else "differs from the oracle"
+    scene.close()
+    return out
+
+
 # ---------------------------------------------------------------------------
 # the rows either side of the path
 # ---------------------------------------------------------------------------
@@ -397,6 +442,7 @@ def main() -> int:
     ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent-ray leg")
     ap.add_argument("--no-rows", action="store_true", help="skip the ingest/shading/LOD leg")
+    ap.add_argument("--urban", action="store_true", help="add the 1024^3 urban leg (primary + bounce rays)")
     ap.add_argument("--vixen-json", default=None, help="also write the headline frames in VIXEN's results schema")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -490,6 +536,10 @@ def main() -> int:
                                   check=not args.no_cpu)
         log(f"[bench] rows: {result['rows']}")
     scene.close()
+    if args.urban:
+        log("[bench] urban leg...")
+        result["urban"] = urban_leg(vx, orc, torch, ctx, stream, max(5, args.steps // 5), check=not args.no_cpu)
+        log(f"[bench] urban: {result['urban']}")
     if not args.no_batched:
         log("[bench] batched leg...")
         result["batched"] = batched_leg(vx, torch, ctx, stream, max(3, args.steps // 10))
