@@ -4,13 +4,22 @@ Python mirror of the reference's ray-cast surface over the C ABI in
 ``libvixen_hip.so`` (declared in ``include/vixen_hip.h`` and
 ``include/vixen_build.h``):
 
-* ``build_from_dense`` / ``gen_*`` -- the scene build path
+* ``build_from_dense`` / ``build_from_bricks`` / ``gen_*`` -- the scene build path
   (``LaineKarrasOctree::rebuild``, libraries/SVO/src/SVORebuild.cpp:228-732, and
   ``VoxelSceneCacher::BuildOctree``, libraries/CashSystem/src/VoxelSceneCacher.cpp:463-628);
+  ``read_cache`` / ``write_cache`` -- VIXEN's VSCA v1 scene cache
+  (libraries/CashSystem/src/VoxelSceneCacher.cpp:179-364);
 * ``Scene.trace`` -- the batch form of ``ISVOStructure::castRay``
   (libraries/SVO/include/ISVOStructure.h:122-126);
 * ``Scene.render`` -- the compute pipeline's per-frame dispatch
   (shaders/VoxelRayMarch_Compressed.comp:493-594);
+* ``Scene.trace_lod`` / ``lod_from_camera`` -- ``castRayWithLOD`` and its
+  ``LODParameters`` (libraries/SVO/include/SVOLOD.h:50-195);
+* ``Batch`` -- every scene of a set in one persistent launch, with its
+  autotuner and hipGraph capture;
+* ``Scene.albedo_forward`` / ``albedo_backward`` -- the differentiable albedo
+  render and its hand-written adjoint (``vx_torch.so`` wraps them in a
+  ``torch.autograd.Function``);
 * ``LaineKarrasOctreeHip.castRay`` -- one-ray convenience with the reference's
   argument meaning (origin, direction, tMin, tMax) returning a ``RayHit``.
 
