@@ -15,7 +15,8 @@
  *     the text of the last failure on the calling thread is vx_last_error();
  *   - invalid rays (NaN/Inf, zero direction) produce a miss, never an error
  *     (libraries/SVO/src/SVOTraversal.cpp:238-259);
- *   - a context owns one device; scenes own their device copies; every call
+ *   - a context names one device; scenes own their device copies and may
+ *     outlive the context they were created in; every call
  *     on a scene, batch or graph runs on its context's device and leaves the
  *     calling thread's current device as it found it (vx_camera_rays, which
  *     takes no scene, runs on the current device);

@@ -126,3 +126,24 @@ def test_maximum_size_sparse_world_bit_exact(vx, orc, gpu_ctx, N):
         hits, _ = scene.render(cam, 258, 129, tmax=1e30)
         torch.cuda.synchronize()
         _check(vx.hits_to_numpy(hits), cwant, f"N={N} camera, lds depth {depth}")
+
+
+def test_scene_and_batch_outlive_their_context(vx, orc, edge_scenes):
+    """Scenes and batches copy their context's device: destroying the context
+    first leaves them usable and destroyable."""
+    import torch
+    grid, built = edge_scenes["full64"]
+    ctx = vx.Context(0)
+    scene = vx.Scene(ctx, built)
+    batch = vx.Batch(ctx, [scene])
+    ctx.close()
+    rays = _rays(orc, np.random.default_rng(3), 4096, built.N)
+    d_rays = vx.rays_to_device(rays)
+    got = vx.hits_to_numpy(scene.trace(d_rays))
+    hits = batch.trace(d_rays, np.array([0, len(rays)], np.uint64))
+    torch.cuda.synchronize()
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    _check(got, want, "scene after its context")
+    _check(vx.hits_to_numpy(hits), want, "batch after its context")
+    batch.close()
+    scene.close()

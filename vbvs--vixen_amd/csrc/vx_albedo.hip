@@ -438,7 +438,7 @@ extern "C" {
 vx_status vx_scene_voxel_count(vx_scene* s, uint64_t* n_voxels)
 {
     if (!s || !n_voxels) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_voxel_count: null argument");
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     vx_status st;
     vx_albedo_state* a = vx::albedo_state(s, nullptr, st);
     if (!a) return st;
@@ -453,7 +453,7 @@ vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint3
         return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: bad arguments");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: image too large");
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     vx_status status;
     vx_albedo_state* a = vx::albedo_state(s, st, status);
@@ -476,7 +476,7 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
         return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: bad arguments");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: image too large");
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     vx_status status;
     vx_albedo_state* a = vx::albedo_state(s, st, status);
@@ -533,7 +533,7 @@ vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint
         return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run selection failed");
     // resident groups walk all runs (their count stays on the device); the keys
     // buffer, consumed by the sort, holds the list of long runs
-    const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->ctx->cu_count * 8u,
+    const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->cu_count * 8u,
                                                                                (px + vx::kAlbBlock - 1) / vx::kAlbBlock));
     vx::k_clear_long_count<<<1, 1, 0, st>>>(a->n_runs);
     vx::k_reduce_short_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals, a->pix_sorted, a->heads, a->n_runs,

@@ -542,7 +542,8 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
 
     auto* s = new (std::nothrow) vx_scene{};
     if (!s) return fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: out of memory");
-    s->ctx = ctx;
+    s->device = ctx->device;
+    s->cu_count = ctx->cu_count;
     s->node_count = d->node_count;
     s->brick_count = d->brick_count;
     s->cfg = c;
@@ -643,7 +644,7 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
 {
     if (!s || (n && (!rays || !hits))) return fail(VX_ERR_INVALID_ARG, "vx_trace: null argument");
     if (n == 0) return VX_OK;
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     const uint64_t chunks = (n + kBlock - 1) / kBlock;
     const uint64_t per = (chunks + kXcds - 1) / kXcds;
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
@@ -738,7 +739,7 @@ vx_status vx_scene_validate(const vx_scene_desc* desc)
 vx_status vx_scene_destroy(vx_scene* s)
 {
     if (!s) return VX_OK;
-    const vx::DeviceGuard guard(s->ctx->device); // frees proceed even if it fails
+    const vx::DeviceGuard guard(s->device); // frees proceed even if it fails
     vx_albedo_state_free(s->albedo);
     if (!s->borrowed) {
         (void)hipFree(s->nodes);
@@ -816,7 +817,7 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     if (!s || !cam || (!hits && !rgba8)) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: null argument");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: image too large");
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     const vx::CamParams c = vx::make_cam(cam);
     const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;
     const uint32_t per = (ntiles + vx::kXcds - 1) / vx::kXcds;
@@ -847,7 +848,7 @@ vx_status vx_bounce_rays(const vx_scene* s, const vx_ray* primary, const vx_hit*
     if (!s || (n && (!primary || !hits || !out))) return vx::fail(VX_ERR_INVALID_ARG, "vx_bounce_rays: null argument");
     if (n == 0) return VX_OK;
     if (n > 0xFFFFFFFFull * vx::kBlock) return vx::fail(VX_ERR_INVALID_ARG, "vx_bounce_rays: too many rays");
-    VX_DEVICE_GUARD(s->ctx->device);
+    VX_DEVICE_GUARD(s->device);
     vx::k_bounce_rays<<<dim3((unsigned)((n + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0,
                         static_cast<hipStream_t>(stream)>>>(primary, hits, n, seed, s->view.k, out);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_bounce_rays: launch failed");

@@ -180,7 +180,8 @@ __global__ void __launch_bounds__(kMaxGroup) __attribute__((amdgpu_waves_per_eu(
 } // namespace vx
 
 struct vx_batch {
-    vx_ctx* ctx;
+    int device;   // copied from the context, as for vx_scene
+    int cu_count;
     uint32_t n;
     std::vector<vx::SceneView> views;     // slices of the concatenated pools
     std::vector<uint32_t> level_prefix;   // n x 4
@@ -267,7 +268,7 @@ static vx_status grid_size(const vx_batch* b, const vx_tuning& t, bool staged, u
     if (staged) VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<true>, block, lds_bytes));
     else VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<false>, block, lds_bytes));
     if (per_cu < 1) return fail(VX_ERR_UNSUPPORTED, "persistent kernel does not fit one workgroup per CU");
-    groups = static_cast<unsigned>(per_cu * b->ctx->cu_count);
+    groups = static_cast<unsigned>(per_cu * b->cu_count);
     return VX_OK;
 }
 
@@ -293,7 +294,7 @@ static vx_status trace_batched(vx_batch* b, const vx_ray* rays, const uint64_t* 
 {
     if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null argument");
     if (off[b->n] > off[0] && (!rays || !hits)) return fail(VX_ERR_INVALID_ARG, "vx_trace_batched: null rays/hits");
-    VX_DEVICE_GUARD(b->ctx->device);
+    VX_DEVICE_GUARD(b->device);
     const vx_tuning t = sanitize(tuning, b);
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
@@ -318,13 +319,18 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
     uint64_t nodes = 0, bricks = 0, bc1_bricks = 0;
     for (uint32_t s = 0; s < n; ++s) {
         if (!scenes[s]) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: null scene");
+        if (scenes[s]->device != ctx->device)
+            return fail(VX_ERR_INVALID_ARG, "vx_batch_create: scene " + std::to_string(s) + " lives on device " +
+                                                std::to_string(scenes[s]->device) + ", the context on " +
+                                                std::to_string(ctx->device));
         nodes += scenes[s]->node_count;
         bricks += scenes[s]->brick_count;
         if (scenes[s]->view.bc1) bc1_bricks += scenes[s]->brick_count;
     }
     auto* b = new (std::nothrow) vx_batch{};
     if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: out of memory");
-    b->ctx = ctx;
+    b->device = ctx->device;
+    b->cu_count = ctx->cu_count;
     b->n = n;
     auto cleanup = [&](vx_status st) {
         (void)hipFree(b->nodes);
@@ -380,7 +386,7 @@ static vx_status autotune(vx_batch* b, const vx_ray* rays, const uint64_t* off, 
                           float* best_ms, hipStream_t st)
 {
     if (!b || !off) return fail(VX_ERR_INVALID_ARG, "vx_batch_autotune: null argument");
-    VX_DEVICE_GUARD(b->ctx->device);
+    VX_DEVICE_GUARD(b->device);
     hipEvent_t e0, e1;
     VX_HIP(hipEventCreate(&e0));
     VX_HIP(hipEventCreate(&e1));
@@ -427,7 +433,7 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
 {
     if (!b || !off || !out) return fail(VX_ERR_INVALID_ARG, "vx_batch_graph_create: null argument");
     *out = nullptr;
-    VX_DEVICE_GUARD(b->ctx->device);
+    VX_DEVICE_GUARD(b->device);
     const vx_tuning t = sanitize(tuning, b);
     std::vector<BatchSceneInfo> tab;
     uint64_t n_chunks = 0;
@@ -436,7 +442,7 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
     if (vx_status s = make_table(b, off, t, tab, n_chunks, lds_bytes, staged); s != VX_OK) return s;
     auto* g = new (std::nothrow) vx_graph{};
     if (!g) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_graph_create: out of memory");
-    g->device = b->ctx->device;
+    g->device = b->device;
     auto cleanup = [&](vx_status st) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -487,7 +493,7 @@ vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scene
 vx_status vx_batch_destroy(vx_batch* b)
 {
     if (!b) return VX_OK;
-    const vx::DeviceGuard guard(b->ctx->device); // frees proceed even if it fails
+    const vx::DeviceGuard guard(b->device); // frees proceed even if it fails
     (void)hipFree(b->nodes);
     (void)hipFree(b->brick_data);
     (void)hipFree(b->occ);

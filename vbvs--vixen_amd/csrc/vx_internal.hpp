@@ -19,8 +19,11 @@ struct vx_ctx {
     int cu_count;
 };
 
+// Scenes and batches copy what they need of their context (its device and CU
+// count), so a context may be destroyed before them.
 struct vx_scene {
-    vx_ctx* ctx;
+    int device;
+    int cu_count;
     uint2* nodes;
     uint32_t* brick_data;
     uint32_t* occ;
