@@ -190,7 +190,12 @@ vx_status vx_scene_create_device(vx_ctx* ctx, const vx_scene_desc* desc, vx_scen
 
 /* The checks vx_scene_create runs before uploading, without touching a device. */
 vx_status vx_scene_validate(const vx_scene_desc* desc);
+/* Drops a reference to the scene; the last one frees it. */
 vx_status vx_scene_destroy(vx_scene* scene);
+/* Takes one more reference (no reference counterpart): a holder that must keep
+ * the scene usable after its creator destroys it, such as an autograd graph
+ * waiting for its backward pass, retains it and calls vx_scene_destroy when done. */
+vx_status vx_scene_retain(vx_scene* scene);
 
 /* Descriptor/brick counts and tree shape of an uploaded scene. */
 vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* brick_count,

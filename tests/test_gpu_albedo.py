@@ -168,3 +168,26 @@ def test_torch_autograd_function(vx, c5):
     loss.backward()
     torch.cuda.synchronize()
     assert alb.grad.cpu().numpy().tobytes() == _gpu_grad(vx, c5).tobytes()
+
+
+def test_autograd_graph_keeps_its_scene(vx, c5, cave, gpu_ctx):
+    """The autograd node holds a reference to the scene (vx_scene_retain): the
+    caller may close its scene between forward and backward, and the backward
+    still runs on it and gives the same gradient; the scene is freed with the graph."""
+    import gc
+
+    import torch
+    scene = vx.Scene(gpu_ctx, cave[2])
+    alb = c5["albedo"].clone().requires_grad_(True)
+    loss = 0.0
+    for v in c5["views"]:
+        img = vx.albedo_render(scene, alb, v["cam"], W, H, c5["light"])
+        loss = loss + ((img - v["target"]) ** 2).sum()
+    scene.close()
+    del scene, img
+    gc.collect()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert alb.grad.cpu().numpy().tobytes() == _gpu_grad(vx, c5).tobytes()
+    del loss
+    gc.collect()

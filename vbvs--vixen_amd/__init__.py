@@ -153,6 +153,7 @@ def _load() -> C.CDLL:
         "vx_ctx_destroy": ([vp], st),
         "vx_scene_create": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_validate": ([C.POINTER(VxSceneDesc)], st),
+        "vx_scene_retain": ([vp], st),
         "vx_scene_create_device": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_destroy": ([vp], st),
         "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
@@ -203,6 +204,7 @@ LIB = _load()
 
 # Every symbol the headers declare (the CPU suite checks the library exports them all).
 EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "vx_scene_create", "vx_scene_validate",
+            "vx_scene_retain",
             "vx_scene_destroy", "vx_scene_info", "vx_trace", "vx_trace_counters", "vx_render",
             "vx_camera_rays", "vx_build_from_dense", "vx_build_from_bricks", "vx_built_scene_free", "vx_bc1_encode_block",
             "vx_gen_cornell", "vx_gen_cave", "vx_gen_urban", "vx_scene_set_lds_depth", "vx_batch_create",

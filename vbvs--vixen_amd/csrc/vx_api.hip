@@ -736,9 +736,17 @@ vx_status vx_scene_validate(const vx_scene_desc* desc)
     }
 }
 
+vx_status vx_scene_retain(vx_scene* s)
+{
+    if (!s) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_retain: null scene");
+    s->refs.fetch_add(1, std::memory_order_relaxed);
+    return VX_OK;
+}
+
 vx_status vx_scene_destroy(vx_scene* s)
 {
     if (!s) return VX_OK;
+    if (s->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return VX_OK; // other references remain
     const vx::DeviceGuard guard(s->device); // frees proceed even if it fails
     vx_albedo_state_free(s->albedo);
     if (!s->borrowed) {

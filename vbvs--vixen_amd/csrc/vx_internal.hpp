@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,9 @@ struct vx_scene {
     // vx_scene_create_device: nodes, brick_data, bc1 and nrm are the caller's
     // device buffers (not freed here); occ and box are always the scene's own
     bool borrowed = false;
+    // references: the creator's, plus one per vx_scene_retain; vx_scene_destroy
+    // drops one and the last frees the scene
+    std::atomic<int> refs{1};
 };
 
 namespace vx {

@@ -36,6 +36,23 @@ vx_camera to_camera(const torch::Tensor& cam)
     return c;
 }
 
+// A one-byte CPU tensor whose storage holds a reference to the scene. Kept in
+// the autograd node, it keeps the scene usable until the node is released,
+// whether or not its backward ever runs, even if the caller destroys the scene
+// first.
+torch::Tensor scene_keepalive(vx_scene* sc)
+{
+    check(vx_scene_retain(sc), "vx_scene_retain");
+    auto* token = new uint8_t(0);
+    return torch::from_blob(
+        token, {1},
+        [sc](void* p) {
+            delete static_cast<uint8_t*>(p);
+            (void)vx_scene_destroy(sc);
+        },
+        torch::TensorOptions().dtype(torch::kUInt8));
+}
+
 struct AlbedoRender : public torch::autograd::Function<AlbedoRender> {
     static torch::Tensor forward(AutogradContext* ctx, torch::Tensor albedo, int64_t scene, torch::Tensor camera,
                                  int64_t width, int64_t height, torch::Tensor light, bool store_hits,
@@ -67,6 +84,7 @@ struct AlbedoRender : public torch::autograd::Function<AlbedoRender> {
               "vx_albedo_forward");
         ctx->save_for_backward({rows, shade, camera, lt});
         ctx->saved_data["scene"] = scene;
+        ctx->saved_data["scene_ref"] = scene_keepalive(sc);
         ctx->saved_data["width"] = width;
         ctx->saved_data["height"] = height;
         ctx->saved_data["voxels"] = static_cast<int64_t>(voxels);
