@@ -305,6 +305,8 @@ typedef struct vx_tuning {
 /* Concatenates the scenes' pools (device-to-device copies) with per-scene
  * offsets. The scenes may be destroyed afterwards. */
 vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scenes, vx_batch** out);
+/* Drops the caller's reference; graphs captured from the batch hold their own
+ * until vx_graph_destroy, and the last reference frees it. */
 vx_status vx_batch_destroy(vx_batch* batch);
 
 /* Traces the rays of all scenes in one persistent launch: scene s owns
@@ -321,7 +323,8 @@ vx_status vx_batch_autotune(vx_batch* batch, const vx_ray* rays, const uint64_t*
                             vx_tuning* best, float* best_ms, vx_stream stream);
 
 /* Captures one batched launch (queue reset + persistent kernel) in a hipGraph
- * for per-frame replay on fixed buffers. */
+ * for per-frame replay on fixed buffers. The graph references the batch, which
+ * stays alive until the graph is destroyed. */
 vx_status vx_batch_graph_create(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
                                 const vx_tuning* tuning, vx_graph** out);
 vx_status vx_graph_launch(vx_graph* graph, vx_stream stream);

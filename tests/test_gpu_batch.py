@@ -113,3 +113,21 @@ def test_graph_replays_match_eager(vx, batch_setup):
         x = torch.randn(1 << 20, device=hits.device)
         (x * 2).sum().item()
     g.close()
+
+
+def test_graph_keeps_its_batch(vx, gpu_ctx, batch_setup):
+    """A captured graph references its batch: the batch may be destroyed first
+    and the graph still replays bit-identically; the last release frees it."""
+    import torch
+    st = batch_setup
+    batch = vx.Batch(gpu_ctx, st["scenes"])
+    hits = torch.zeros_like(st["d_rays"])
+    g = batch.graph(st["d_rays"], st["offsets"], hits, tuning={"waves_per_group": 2, "chunk": 256})
+    batch.close()
+    for _ in range(2):
+        hits.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert hits.cpu().numpy().tobytes() == st["plain"].tobytes()
+    g.close()

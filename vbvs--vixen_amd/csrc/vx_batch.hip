@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -195,10 +196,15 @@ struct vx_batch {
     unsigned int* d_queue = nullptr;
     std::vector<vx::BatchSceneInfo> h_table;
     vx_tuning tuned{0, 0, 0, 0};
+    // references: the creator's plus one per graph captured from it (a graph's
+    // kernels read the batch's pools); the last vx_batch_destroy or
+    // vx_graph_destroy frees it
+    std::atomic<int> refs{1};
 };
 
 struct vx_graph {
-    int device = 0; // the batch context's device
+    int device = 0;             // the batch context's device
+    vx_batch* batch = nullptr;  // referenced until the graph is destroyed
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     vx::BatchSceneInfo* d_table = nullptr;
@@ -473,6 +479,8 @@ static vx_status graph_create(vx_batch* b, const vx_ray* rays, const uint64_t* o
     }
     (void)hipStreamDestroy(cs);
     if (st != VX_OK) return cleanup(st);
+    b->refs.fetch_add(1, std::memory_order_relaxed);
+    g->batch = b;
     *out = g;
     return VX_OK;
 }
@@ -493,6 +501,7 @@ vx_status vx_batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n_scene
 vx_status vx_batch_destroy(vx_batch* b)
 {
     if (!b) return VX_OK;
+    if (b->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return VX_OK; // a graph still uses it
     const vx::DeviceGuard guard(b->device); // frees proceed even if it fails
     (void)hipFree(b->nodes);
     (void)hipFree(b->brick_data);
@@ -552,8 +561,9 @@ vx_status vx_graph_destroy(vx_graph* g)
     if (g->graph) (void)hipGraphDestroy(g->graph);
     (void)hipFree(g->d_table);
     (void)hipFree(g->d_queue);
+    vx_batch* b = g->batch;
     delete g;
-    return VX_OK;
+    return vx_batch_destroy(b); // the graph's reference
 }
 
 } // extern "C"
