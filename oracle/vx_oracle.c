@@ -509,6 +509,7 @@ static void trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hi
     if (nthreads > 256) nthreads = 256;
     orc_job jobs[256];
     pthread_t th[256];
+    int started[256];
     const uint64_t per = (n + (uint64_t)nthreads - 1) / (uint64_t)nthreads;
     for (int t = 0; t < nthreads; ++t) {
         jobs[t].s = s;
@@ -517,10 +518,11 @@ static void trace_mt(const orc_scene* s, const orc_ray* rays, uint64_t n, orc_hi
         jobs[t].lod = lod;
         jobs[t].begin = per * (uint64_t)t < n ? per * (uint64_t)t : n;
         jobs[t].end = per * (uint64_t)(t + 1) < n ? per * (uint64_t)(t + 1) : n;
-        pthread_create(&th[t], NULL, orc_worker, &jobs[t]);
+        started[t] = pthread_create(&th[t], NULL, orc_worker, &jobs[t]) == 0;
+        if (!started[t]) orc_worker(&jobs[t]); /* no thread: run the range here */
     }
     for (int t = 0; t < nthreads; ++t) {
-        pthread_join(th[t], NULL);
+        if (started[t]) pthread_join(th[t], NULL);
         if (counts)
             for (int k = 0; k < 5; ++k) counts[k] += jobs[t].c[k];
     }
