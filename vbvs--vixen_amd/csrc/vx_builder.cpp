@@ -18,6 +18,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -64,7 +65,11 @@ static void parallel_for(uint64_t n, int nthreads, F&& f)
     const uint64_t per = (n + nt - 1) / nt;
     for (uint64_t t = 0; t < nt; ++t) {
         const uint64_t b = std::min(n, t * per), e = std::min(n, (t + 1) * per);
-        th.emplace_back([&f, b, e] { f(b, e); });
+        try {
+            th.emplace_back([&f, b, e] { f(b, e); });
+        } catch (const std::system_error&) {
+            f(b, e); // no thread available: run the range here
+        }
     }
     for (auto& t : th) t.join();
 }

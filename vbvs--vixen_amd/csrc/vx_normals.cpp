@@ -22,6 +22,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -247,9 +248,14 @@ vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, in
     std::vector<std::thread> th;
     for (uint64_t t = 0; t < nt; ++t) {
         const uint64_t b = std::min(brick_count, t * per), e = std::min(brick_count, (t + 1) * per);
-        th.emplace_back([=] {
+        auto run = [=] {
             for (uint64_t i = b; i < e; ++i) vx::encode_brick(brick_data + i * 512, normals + i * 64);
-        });
+        };
+        try {
+            th.emplace_back(run);
+        } catch (const std::system_error&) {
+            run(); // no thread available: run the range here
+        }
     }
     for (auto& x : th) x.join();
     return VX_OK;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -54,10 +55,16 @@ static void par_z(uint32_t N, F&& f)
 {
     const unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 64u));
     std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
+    for (unsigned t = 0; t < nt; ++t) {
+        auto run = [&, t] {
             for (uint32_t z = t; z < N; z += nt) f(z);
-        });
+        };
+        try {
+            th.emplace_back(run);
+        } catch (const std::system_error&) {
+            run(); // no thread available: run the slices here
+        }
+    }
     for (auto& x : th) x.join();
 }
 
