@@ -17,6 +17,7 @@
 // glm::normalize(v) is v * (1 / sqrt(dot(v, v))) and glm::length(v) is
 // sqrt(dot(v, v)), dot summed left to right.
 #include "vixen_build.h"
+#include "vx_common.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -241,7 +242,8 @@ extern "C" {
 
 vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, int nthreads, uint64_t* normals)
 {
-    if (brick_count && (!brick_data || !normals)) return VX_ERR_INVALID_ARG;
+    if (brick_count && (!brick_data || !normals))
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_encode_normals: null argument");
     if (nthreads <= 0) nthreads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
     const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(nthreads), brick_count / 64 + 1));
     const uint64_t per = (brick_count + nt - 1) / nt;
