@@ -116,7 +116,7 @@ vx_status read_file(const char* path, std::vector<uint8_t>& bytes)
 
 extern "C" {
 
-vx_status vx_cache_read(const char* path, vx_cache** out)
+static vx_status cache_read_impl(const char* path, vx_cache** out)
 {
     if (!path || !out) return vx::fail(VX_ERR_INVALID_ARG, "vx_cache_read: null argument");
     *out = nullptr;
@@ -203,7 +203,7 @@ vx_status vx_cache_free(vx_cache* cache)
     return VX_OK;
 }
 
-vx_status vx_cache_write(const char* path, const vx_cache_entry_in* entries, uint32_t n)
+static vx_status cache_write_impl(const char* path, const vx_cache_entry_in* entries, uint32_t n)
 {
     if (!path || (n && !entries)) return vx::fail(VX_ERR_INVALID_ARG, "vx_cache_write: null argument");
     vx::Writer w;
@@ -241,6 +241,29 @@ vx_status vx_cache_write(const char* path, const vx_cache_entry_in* entries, uin
     const bool closed = std::fclose(f) == 0;
     if (!ok || !closed) return vx::fail(VX_ERR_INVALID_ARG, std::string("vx_cache_write: write failed on ") + path);
     return VX_OK;
+}
+
+// Nothing may throw across the ABI: allocation failures become statuses.
+vx_status vx_cache_read(const char* path, vx_cache** out)
+{
+    try {
+        return cache_read_impl(path, out);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_cache_read: out of memory");
+    } catch (...) {
+        return vx::fail(VX_ERR_INVALID_SCENE, "vx_cache_read: unexpected exception");
+    }
+}
+
+vx_status vx_cache_write(const char* path, const vx_cache_entry_in* entries, uint32_t n)
+{
+    try {
+        return cache_write_impl(path, entries, n);
+    } catch (const std::bad_alloc&) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_cache_write: out of memory");
+    } catch (...) {
+        return vx::fail(VX_ERR_INVALID_ARG, "vx_cache_write: unexpected exception");
+    }
 }
 
 } // extern "C"

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <system_error>
 #include <thread>
+#include <new>
 #include <vector>
 
 namespace vx {
@@ -240,7 +241,7 @@ void encode_brick(const uint32_t* brick, uint64_t* out /* 64 u64 */)
 
 extern "C" {
 
-vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, int nthreads, uint64_t* normals)
+static vx_status encode_normals_impl(const uint32_t* brick_data, uint64_t brick_count, int nthreads, uint64_t* normals)
 {
     if (brick_count && (!brick_data || !normals))
         return vx::fail(VX_ERR_INVALID_ARG, "vx_encode_normals: null argument");
@@ -261,6 +262,15 @@ vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, in
     }
     for (auto& x : th) x.join();
     return VX_OK;
+}
+
+vx_status vx_encode_normals(const uint32_t* brick_data, uint64_t brick_count, int nthreads, uint64_t* normals)
+{
+    try {
+        return encode_normals_impl(brick_data, brick_count, nthreads, normals);
+    } catch (...) {
+        return vx::fail(VX_ERR_OUT_OF_MEMORY, "vx_encode_normals: out of memory");
+    }
 }
 
 void vx_dxt_normal_encode_block(const float normals[16][3], uint64_t out[2])
