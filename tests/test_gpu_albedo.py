@@ -191,3 +191,21 @@ def test_autograd_graph_keeps_its_scene(vx, c5, cave, gpu_ctx):
     assert alb.grad.cpu().numpy().tobytes() == _gpu_grad(vx, c5).tobytes()
     del loss
     gc.collect()
+
+
+@pytest.mark.parametrize("fraction", [0.25, 0.0])
+def test_automatic_store_or_retrace(vx, c5, fraction):
+    """store_hits="auto" stores the hits at the default budget and re-traces them
+    when no memory may be spent (fraction 0); the gradient is the same either way."""
+    import torch
+    free = vx.free_device_bytes(c5["albedo"].device)
+    assert vx.stores_hits(W, H, free, fraction) == (fraction > 0.0)
+    alb = c5["albedo"].clone().requires_grad_(True)
+    loss = 0.0
+    for v in c5["views"]:
+        img = vx.albedo_render(c5["scene"], alb, v["cam"], W, H, c5["light"], store_hits="auto",
+                               memory_fraction=fraction)
+        loss = loss + ((img - v["target"]) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert alb.grad.cpu().numpy().tobytes() == _gpu_grad(vx, c5).tobytes()

@@ -758,10 +758,41 @@ def torch_ext():
     return _TORCH_EXT
 
 
-def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, store_hits: bool = True,
-                  deterministic: bool = True, stream=None):
-    """Differentiable render (torch::autograd::Function in vx_torch.so): image [h,w,3]; albedo gets gradients."""
+# Bytes per pixel the forward keeps for the backward when it stores its hits:
+# the hit voxel's albedo row (u32) and its shade (f32).
+HIT_STASH_BYTES_PER_PIXEL = 8
+
+
+def stores_hits(w: int, h: int, free_bytes: int, memory_fraction: float = 0.25) -> bool:
+    """The automatic choice between storing the forward's hits and re-tracing
+    them in the backward: store while the w x h stash takes at most
+    memory_fraction of the device memory still free, re-trace when memory is tight."""
+    if not 0.0 <= memory_fraction <= 1.0:
+        raise ValueError(f"memory_fraction must lie in [0, 1], got {memory_fraction}")
+    return HIT_STASH_BYTES_PER_PIXEL * int(w) * int(h) <= memory_fraction * int(free_bytes)
+
+
+def free_device_bytes(device) -> int:
+    """Device memory a new allocation can take: what the driver reports free plus
+    what torch's caching allocator holds reserved but unused."""
     torch = _torch()
+    free, _ = torch.cuda.mem_get_info(device)
+    return int(free) + int(torch.cuda.memory_reserved(device)) - int(torch.cuda.memory_allocated(device))
+
+
+def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, store_hits="auto",
+                  deterministic: bool = True, stream=None, memory_fraction: float = 0.25):
+    """Differentiable render (torch::autograd::Function in vx_torch.so): image [h,w,3]; albedo gets gradients.
+
+    store_hits: True keeps each pixel's albedo row and shade (8 bytes) for the
+    backward, False makes the backward re-trace the hits (same gradient, bit for
+    bit), "auto" (the default) stores unless that stash would exceed
+    memory_fraction of the free device memory (stores_hits)."""
+    torch = _torch()
+    if isinstance(store_hits, str):
+        if store_hits != "auto":
+            raise ValueError(f"store_hits must be True, False or 'auto', got {store_hits!r}")
+        store_hits = stores_hits(w, h, free_device_bytes(albedo.device), memory_fraction)
     camt = torch.from_numpy(cam.as_floats())
     lt = torch.tensor(np.asarray(light, dtype=np.float32))
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -769,9 +800,12 @@ def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, 
                                      bool(deterministic), int(s.cuda_stream))
 
 
-def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None):
+def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None, out=None):
+    """The w x h pinhole rays of a camera (vx_ray, 32 bytes each), into `out` when given."""
     torch = _torch()
-    rays = torch.empty(w * h * 32, dtype=torch.uint8, device=torch.device("cuda", device))
+    if out is not None and out.numel() * out.element_size() < w * h * 32:
+        raise ValueError(f"out holds {out.numel() * out.element_size()} bytes, {w * h * 32} needed")
+    rays = out if out is not None else torch.empty(w * h * 32, dtype=torch.uint8, device=torch.device("cuda", device))
     _check(LIB.vx_camera_rays(C.byref(cam), w, h, float(tmax), rays.data_ptr(), _stream_ptr(stream)),
            "vx_camera_rays")
     return rays
