@@ -22,7 +22,10 @@
  *     takes no scene, runs on the current device);
  *   - trace/render calls are asynchronous on the stream they are given and
  *     take DEVICE pointers for rays, hits and images (inputs already resident
- *     in HBM); scene creation takes HOST pointers and copies.
+ *     in HBM); scene creation takes HOST pointers and copies;
+ *   - vx_trace, vx_trace_lod, vx_render and vx_camera_rays only launch kernels
+ *     (no allocation, memset or synchronisation), so a caller may record them
+ *     into a HIP graph with stream capture and replay the frame.
  */
 #ifndef VIXEN_HIP_H
 #define VIXEN_HIP_H
