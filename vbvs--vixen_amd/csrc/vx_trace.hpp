@@ -700,15 +700,17 @@ template <bool COUNT, bool STAGED, bool IMMEDIATE, bool LOD = false>
 __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceCounts& cnt, LeafRef& lf)
 {
     const int iter = r.iter;
-    if (!(iter < MAX_ITERS && r.scale <= ESVO_MAX_SCALE)) {
+    // The reference loop also ends when scale exceeds the root's; here POP
+    // finishes such a ray itself, so the scale is always <= ESVO_MAX_SCALE here.
+    if (iter >= MAX_ITERS) {
         rec_miss(s, (uint32_t)iter);
         return STEP_DONE;
     }
     if (COUNT) cnt.iters += 1;
-    if (r.parent >= s.node_count) {
-        rec_miss(s, (uint32_t)(iter + 1));
-        return STEP_DONE;
-    }
+    // Every scene is validated at creation (vx_scene_create walks the tree from
+    // the root): each child block lies inside the descriptor array and each brick
+    // id is either 0xFFFFFF or a valid brick, so the parent, the leaf descriptor
+    // index and the brick id below need no range checks.
     const uint2 desc = fetch_node<STAGED>(s, r.parent);
     const uint32_t valid = (desc.x >> 16) & 0xFFu, leaf = (desc.x >> 24) & 0xFFu;
     const uint32_t cp = (desc.x & 0x8000u) ? (desc.y & 0xFFFFFFu) : (desc.x & 0x7FFFu);
@@ -745,17 +747,15 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
             if (COUNT) cnt.leaves += 1;
             const uint32_t below = (1u << local) - 1u;
             const uint32_t lidx = cp + popc8(valid & ~leaf) + popc8(valid & leaf & below);
-            if (lidx < s.node_count) {
-                const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
-                if (brick != 0xFFFFFFu && brick < s.brick_count) {
-                    if (IMMEDIATE) {
-                        int bc[3];
-                        brick_cell(s, r, bc);
-                        if (visit_leaf<COUNT>(s, r, brick, bc, lidx, cnt)) return STEP_DONE;
-                    } else {
-                        lf = LeafRef{brick, lidx};
-                        return STEP_LEAF;
-                    }
+            const uint32_t brick = fetch_node<STAGED>(s, lidx).y & 0xFFFFFFu;
+            if (brick != 0xFFFFFFu) {
+                if (IMMEDIATE) {
+                    int bc[3];
+                    brick_cell(s, r, bc);
+                    if (visit_leaf<COUNT>(s, r, brick, bc, lidx, cnt)) return STEP_DONE;
+                } else {
+                    lf = LeafRef{brick, lidx};
+                    return STEP_LEAF;
                 }
             }
             // no hit in the leaf (or no usable brick): the reference goes on to
