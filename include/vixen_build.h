@@ -17,8 +17,10 @@
  *   - BC1 blocks cover 16 consecutive linear voxels, empty voxels encoded as
  *     black, all-empty blocks stored as 0 (SVORebuild.cpp:613-731, encoder
  *     libraries/VoxelData/src/Compression/DXT1Compressor.cpp:38-111);
- *   - the 256-byte OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it, with
- *     grid bounds [0,N]^3 in voxel units.
+ *   - the 256-byte OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it: grid
+ *     bounds [0,N]^3 (the voxel units vx_scene_create traces in), localToWorld
+ *     = scale(10) and worldToLocal its inverse scale(0.1), the compute shader's
+ *     10-unit world (WORLD_GRID_SIZE, :605-625).
  * Far pointers: a child block index above 0x7FFF sets farBit and is stored in
  * word1 bits 0-23 (the contour field, unused on this path).
  */
@@ -109,6 +111,8 @@ typedef struct vx_cache_entry_info {
     uint64_t material_count;
     const uint32_t* brick_grid_lookup;
     uint64_t brick_grid_lookup_count;
+    float world_grid_size;       /* OctreeConfig::worldGridSize, the 4 bytes after the
+                                    256-byte UBO (10 in every file VIXEN writes) */
 } vx_cache_entry_info;
 
 /* Reads a cache file. Errors follow the reference loader's cases: no file ->
@@ -136,9 +140,20 @@ typedef struct vx_cache_entry_in {
     const uint32_t* brick_grid_lookup; /* or NULL */
     uint64_t brick_grid_lookup_count;
     uint32_t solid_voxel_count;
+    float world_grid_size;       /* written after the config; a value that is not
+                                    finite and positive (such as 0) writes VIXEN's 10 */
 } vx_cache_entry_in;
 
-/* Writes a VSCA v1 file VIXEN's VoxelSceneCacher::DeserializeFromFile loads. */
+/* Writes a VSCA v1 file VIXEN's VoxelSceneCacher::DeserializeFromFile loads.
+ *
+ * Units. A scene traces in the world box its config's grid_min/grid_max name
+ * (voxel units [0,N]^3 as VIXEN's CPU castRay, VoxelSceneCacher.cpp:526-537).
+ * VIXEN's compute shader instead maps world space to the unit cube with
+ * worldToLocal (VoxelRayMarch_Compressed.comp:352-354), i.e. traces in
+ * [0, worldGridSize]^3. To trace a cached scene with cameras and rays in that
+ * frame, create it from a config copy with grid_max = world_grid_size: the
+ * world -> voxel map is then N / worldGridSize, the shader's for its
+ * scale-only matrices. */
 vx_status vx_cache_write(const char* path, const vx_cache_entry_in* entries, uint32_t n);
 
 /* Synthetic scenes (voxel units, world = [0,N]^3). Each fills grid (N^3 words)

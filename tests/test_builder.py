@@ -172,7 +172,11 @@ def test_config_matches_reference_formulas(vx, cave, cornell):
         assert c.brick_esvo_scale == 22 - (L - 1 - (L - 3))
         assert c.bricks_per_axis == N // 8
         assert list(c.grid_min) == [0, 0, 0] and list(c.grid_max) == [N, N, N]
-        assert c.local_to_world[0] == N and c.world_to_local[0] == 1.0 / N and c.local_to_world[15] == 1.0
+        # localToWorld = scale(WORLD_GRID_SIZE = 10) and its inverse (VoxelSceneCacher.cpp:605-625)
+        ltw = np.frombuffer(bytes(c.local_to_world), np.float32)
+        wtl = np.frombuffer(bytes(c.world_to_local), np.float32)
+        assert np.array_equal(ltw, np.diag([10, 10, 10, 1]).astype(np.float32).ravel())
+        assert np.array_equal(wtl, np.diag(np.float32([0.1, 0.1, 0.1, 1])).ravel())
 
 
 def test_generators_are_seeded(vx):

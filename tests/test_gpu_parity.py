@@ -267,6 +267,47 @@ def test_scene_over_device_buffers_renders_like_copied(vx, orc, cave, gpu_ctx):
         vx.Scene.from_device(gpu_ctx, t["nodes"], t["brick_data"][1:], b.config)
 
 
+def test_vixen_layout_cache_file_traces_like_the_oracle(vx, orc, cave, gpu_ctx, tmp_path):
+    # A VSCA v1 file laid out as VoxelSceneCacher::SerializeToFile writes it, assembled field by field
+    # (tests/vsca_layout.py: 260-byte OctreeConfig with worldGridSize = 10 at +256, scale(10) matrices,
+    # gridMax = N), loads and traces bit-exact vs the oracle on the loaded buffers: in voxel units (the
+    # config's grid box) and in the compute shader's 10-unit frame.
+    import vsca_layout as L
+    grid, cam, built = cave
+    b = vx.BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count,
+                      built.config).encode_normals()
+    f = tmp_path / "cave.vsca"
+    f.write_bytes(L.file([L.built_entry(vx, b, 2, 256, 0.5, 2)]))
+    (meta, loaded), = vx.read_cache(f)
+    assert meta["world_grid_size"] == 10.0 and list(loaded.config.grid_max) == [256.0] * 3
+    w, h = 640, 360
+    scene = vx.Scene(gpu_ctx, loaded)
+    hits, img = scene.render(cam, w, h, tmax=1e30, rgba=True)
+    _sync()
+    osc = orc.OracleScene.from_built(loaded)
+    rays = orc.camera_rays(cam.as_floats(), w, h, 1e30)
+    want, _ = orc.trace(osc, rays)
+    got = vx.hits_to_numpy(hits)
+    assert len(record_diff(got, want)) == 0
+    assert np.array_equal(img.cpu().numpy().view(np.uint32), orc.render(osc, cam.as_floats(), w, h, 1e30))
+    # the shader's frame: world = 10 / N voxel units; the camera moves with it
+    k = np.float32(10.0) / np.float32(256.0)
+    sf = vx.shader_frame(loaded, meta["world_grid_size"])
+    cam10 = vx.VxCamera.from_buffer_copy(bytes(cam))
+    cam10.pos[:] = [float(np.float32(p) * k) for p in cam.pos]
+    scene10 = vx.Scene(gpu_ctx, sf)
+    h10, _ = scene10.render(cam10, w, h, tmax=1e30)
+    _sync()
+    want10, _ = orc.trace(orc.OracleScene.from_built(sf), orc.camera_rays(cam10.as_floats(), w, h, 1e30))
+    got10 = vx.hits_to_numpy(h10)
+    assert len(record_diff(got10, want10)) == 0
+    # the same surfaces: hit voxels agree with the voxel-frame render except where a ray grazes an edge
+    same = (got10["brick"] == got["brick"]) & (got10["voxel"] == got["voxel"])
+    assert same.mean() > 0.995
+    assert np.allclose(got10["t"][same & (got["flags"] & 1 == 1)] / k,
+                       got["t"][same & (got["flags"] & 1 == 1)], rtol=1e-4)
+
+
 def test_scene_from_cache_file_renders_like_built(vx, orc, cave, gpu_ctx, tmp_path):
     # a VSCA v1 cache round trip (VIXEN's scene cache format) feeds vx_scene_create unchanged
     grid, cam, built = cave

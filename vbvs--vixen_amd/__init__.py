@@ -93,14 +93,15 @@ class VxCacheEntryInfo(C.Structure):
     _fields_ = [("key", C.c_uint64), ("scene_type", C.c_uint32), ("resolution", C.c_uint32), ("density", C.c_float),
                 ("seed", C.c_uint32), ("node_count", C.c_uint32), ("brick_count", C.c_uint32),
                 ("solid_voxel_count", C.c_uint32), ("materials", C.c_void_p), ("material_count", C.c_uint64),
-                ("brick_grid_lookup", C.c_void_p), ("brick_grid_lookup_count", C.c_uint64)]
+                ("brick_grid_lookup", C.c_void_p), ("brick_grid_lookup_count", C.c_uint64),
+                ("world_grid_size", C.c_float)]
 
 
 class VxCacheEntryIn(C.Structure):
     _fields_ = [("scene_type", C.c_uint32), ("resolution", C.c_uint32), ("density", C.c_float), ("seed", C.c_uint32),
                 ("scene", C.POINTER(VxSceneDesc)), ("materials", C.c_void_p), ("material_count", C.c_uint64),
                 ("brick_grid_lookup", C.c_void_p), ("brick_grid_lookup_count", C.c_uint64),
-                ("solid_voxel_count", C.c_uint32)]
+                ("solid_voxel_count", C.c_uint32), ("world_grid_size", C.c_float)]
 
 
 class VxTuning(C.Structure):
@@ -389,12 +390,26 @@ def brick_grid_lookup(built: BuiltScene) -> np.ndarray:
 
 
 SCENE_TYPES = {"cornell": 0, "noise": 1, "tunnels": 2, "cityscape": 3, "custom": 255}
+# VoxelSceneCacher::BuildOctree's WORLD_GRID_SIZE (VoxelSceneCacher.cpp:605-607)
+WORLD_GRID_SIZE = 10.0
+
+
+def shader_frame(built: BuiltScene, world_grid_size: float = WORLD_GRID_SIZE) -> BuiltScene:
+    """The scene with its traced world box set to the compute shader's frame, [0, world_grid_size]^3
+    (VoxelRayMarch_Compressed.comp:352-354 maps world space to the unit cube with worldToLocal =
+    scale(1 / worldGridSize)); the buffers are shared, only the config is copied."""
+    cfg = VxOctreeConfig.from_buffer_copy(bytes(built.config))
+    for i in range(3):
+        cfg.grid_min[i], cfg.grid_max[i] = 0.0, float(world_grid_size)
+    return BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count, cfg,
+                      built.normals)
 
 
 def write_cache(path, entries) -> None:
     """Writes a VSCA v1 scene cache. entries: dicts with built (BuiltScene), scene_type (int or name),
     resolution, density, seed, and optionally materials (bytes/uint8 array of 32-byte records),
-    solid_voxel_count; the brick grid lookup is derived from built.brick_coords."""
+    solid_voxel_count and world_grid_size (default VIXEN's 10, written after the 256-byte config);
+    the brick grid lookup is derived from built.brick_coords."""
     keep, ins = [], (VxCacheEntryIn * max(1, len(entries)))()
     for i, e in enumerate(entries):
         built = e["built"]
@@ -412,6 +427,7 @@ def write_cache(path, entries) -> None:
         ins[i].brick_grid_lookup = _ptr(lookup)
         ins[i].brick_grid_lookup_count = lookup.size
         ins[i].solid_voxel_count = int(e.get("solid_voxel_count", int((built.brick_data != 0).sum())))
+        ins[i].world_grid_size = float(e.get("world_grid_size", WORLD_GRID_SIZE))
         keep.append((desc, cfg, lookup, mats, built))
     _check(LIB.vx_cache_write(str(path).encode(), ins, len(entries)), "vx_cache_write")
 
@@ -455,7 +471,7 @@ def read_cache(path) -> list[tuple[dict, BuiltScene]]:
                     "density": float(info.density), "seed": int(info.seed), "node_count": int(info.node_count),
                     "brick_count": int(info.brick_count), "solid_voxel_count": int(info.solid_voxel_count),
                     "materials": b"" if mats is None else mats.tobytes(),
-                    "brick_grid_lookup": lookup}
+                    "brick_grid_lookup": lookup, "world_grid_size": float(info.world_grid_size)}
             built = BuiltScene(nodes, bricks, None if bc1 is None else bc1.reshape(nb, 32), coords,
                                max(0, nn - nb), cfg, None if nrm is None else nrm.reshape(nb, 64))
             out.append((meta, built))

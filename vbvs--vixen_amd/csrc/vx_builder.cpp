@@ -281,7 +281,10 @@ static vx_status build_tree(const std::vector<uint64_t>& codes, uint32_t N, int 
         }
     });
 
-    // OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it (voxel units).
+    // OctreeConfig as VoxelSceneCacher.cpp:592-625 fills it: grid bounds [0,N]^3
+    // (voxel units), localToWorld = scale(WORLD_GRID_SIZE = 10), worldToLocal its
+    // inverse (the compute shader's 10-unit world; the traversal reads only the
+    // grid bounds).
     vx_octree_config& c = s->config;
     std::memset(&c, 0, sizeof c);
     const int L = levels + 3;
@@ -299,8 +302,8 @@ static vx_status build_tree(const std::vector<uint64_t>& codes, uint32_t N, int 
     }
     for (int i = 0; i < 16; ++i) c.local_to_world[i] = c.world_to_local[i] = 0.0f;
     for (int i = 0; i < 3; ++i) {
-        c.local_to_world[i * 5] = Nf;
-        c.world_to_local[i * 5] = 1.0f / Nf;
+        c.local_to_world[i * 5] = 10.0f;
+        c.world_to_local[i * 5] = 0.1f;
     }
     c.local_to_world[15] = c.world_to_local[15] = 1.0f;
     *out = s;

@@ -12,7 +12,9 @@
 //   six vectors, each u64 element count + elements:
 //     esvoNodes (bytes), brickData (bytes), materials (32-byte GPUMaterial),
 //     compressedColors (bytes), compressedNormals (bytes), brickGridLookup (u32)
-//   OctreeConfig (256 bytes)
+//   OctreeConfig, sizeof(OctreeConfig) = 260 bytes: the 256-byte UBO, then
+//     f32 worldGridSize at +256 (VoxelSceneCacher.h:100-133, written whole at
+//     VoxelSceneCacher.cpp:254 and read whole at :336)
 //   u32 nodeCount, u32 brickCount, u32 solidVoxelCount, u32 resolution, u32 sceneType
 #include "vixen_build.h"
 #include "vx_common.hpp"
@@ -31,6 +33,7 @@ struct vx_cache {
         std::vector<uint8_t> nodes, bricks, materials, colors, normals;
         std::vector<uint32_t> lookup;
         vx_octree_config cfg;
+        float world_grid_size;
         uint32_t node_count, brick_count, solid_voxels, meta_resolution, meta_scene_type;
     };
     std::vector<Entry> entries;
@@ -42,6 +45,9 @@ namespace {
 constexpr uint32_t kMagic = 0x56534341u;
 constexpr uint32_t kVersion = 1u;
 constexpr uint64_t kMaterialBytes = 32;
+// VoxelSceneCacher::BuildOctree's WORLD_GRID_SIZE (VoxelSceneCacher.cpp:605-607)
+constexpr float kWorldGridSize = 10.0f;
+static_assert(sizeof(vx_octree_config) == 256, "the OctreeConfig UBO portion is 256 bytes");
 
 uint64_t cache_key(uint32_t scene_type, uint32_t resolution, float density, uint32_t seed)
 {
@@ -141,8 +147,9 @@ static vx_status cache_read_impl(const char* path, vx_cache** out)
             return vx::fail(VX_ERR_INVALID_SCENE, "vx_cache_read: key does not match the create info" + at);
         }
         ok = ok && r.vec(e.nodes) && r.vec(e.bricks) && r.vec(e.materials, vx::kMaterialBytes) && r.vec(e.colors) &&
-             r.vec(e.normals) && r.vec(e.lookup) && r.get(e.cfg) && r.get(e.node_count) && r.get(e.brick_count) &&
-             r.get(e.solid_voxels) && r.get(e.meta_resolution) && r.get(e.meta_scene_type);
+             r.vec(e.normals) && r.vec(e.lookup) && r.get(e.cfg) && r.get(e.world_grid_size) &&
+             r.get(e.node_count) && r.get(e.brick_count) && r.get(e.solid_voxels) && r.get(e.meta_resolution) &&
+             r.get(e.meta_scene_type);
         if (!ok) {
             delete c;
             return vx::fail(VX_ERR_INVALID_SCENE, "vx_cache_read: truncated file" + at);
@@ -184,6 +191,7 @@ vx_status vx_cache_entry(const vx_cache* cache, uint32_t index, vx_cache_entry_i
         info->material_count = e.materials.size() / vx::kMaterialBytes;
         info->brick_grid_lookup = e.lookup.empty() ? nullptr : e.lookup.data();
         info->brick_grid_lookup_count = e.lookup.size();
+        info->world_grid_size = e.world_grid_size;
     }
     if (desc) {
         desc->esvo_nodes = reinterpret_cast<const uint32_t*>(e.nodes.data());
@@ -229,6 +237,8 @@ static vx_status cache_write_impl(const char* path, const vx_cache_entry_in* ent
         w.vec(d->dxt_normals, d->dxt_normals ? nb * 512 : 0, 1);
         w.vec(e.brick_grid_lookup, e.brick_grid_lookup ? e.brick_grid_lookup_count : 0, 4);
         w.put(*d->config);
+        const float wgs = e.world_grid_size;
+        w.put(wgs > 0.0f && wgs < __builtin_inff() ? wgs : vx::kWorldGridSize); // sizeof(OctreeConfig) = 260
         w.put(static_cast<uint32_t>(d->node_count));
         w.put(static_cast<uint32_t>(nb));
         w.put(e.solid_voxel_count);
