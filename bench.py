@@ -273,26 +273,42 @@ def fwd_bwd_leg(vx, torch, ctx, stream, grid, built, steps: int):
 # ---------------------------------------------------------------------------
 # incoherent-ray leg
 # ---------------------------------------------------------------------------
-def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int):
-    g = torch.Generator(device="cuda").manual_seed(1)
-    n = 2_000_000
-    rr = torch.empty((n, 8), dtype=torch.float32, device="cuda")
-    rr[:, 0:3] = torch.rand((n, 3), generator=g, device="cuda") * N
-    rr[:, 3] = 0.0
-    rr[:, 4:7] = torch.randn((n, 3), generator=g, device="cuda")
-    rr[:, 7] = 3.0e38
-    rr = rr.view(torch.uint8).view(-1)
-    h_rand = torch.empty_like(rr)
-    primary = vx.camera_rays(cam, W, H, 1e30)
-    bounce = scene.bounce_rays(primary, scene.trace(primary), seed=7)
-    h_bounce = torch.empty_like(bounce)
-    ms_rand = timed(torch, stream, lambda: scene.trace(rr, h_rand, stream=stream), steps, 2)
-    ms_bounce = timed(torch, stream, lambda: scene.trace(bounce, h_bounce, stream=stream), steps, 2)
-    return {"workload": "cave256: 2M random rays from inside the volume (seed 1); one cosine bounce ray per "
-                        "primary hit of the 1080p frame (seed 7)",
-            "random_ms": round(ms_rand, 4), "random_mrays": round(n / ms_rand / 1e3, 1),
-            "bounce_ms": round(ms_bounce, 4), "bounce_rays": W * H,
-            "bounce_mrays": round(W * H / ms_bounce / 1e3, 1)}
+def incoherent_leg(vx, torch, scene, cam, W, H, N, stream, steps: int, which: str = "both"):
+    """vx_trace on 2M seeded random rays from inside the volume and/or one seeded cosine bounce ray per
+    primary hit of the headline frame; per ray set its time and the reference algorithm's per-ray counts
+    (vx_trace_counters on the same rays, outside the timed region)."""
+    out = {"workload": "cave256: 2M random rays from inside the volume (seed 1); one cosine bounce ray per "
+                       "primary hit of the 1080p frame (seed 7)"}
+
+    def per_ray(rays, n):
+        _, cnt = scene.trace_counters(rays)
+        torch.cuda.synchronize()
+        c = cnt.cpu().numpy().astype(np.float64)
+        b = 8 * c[1] + 8 * c[2] + 4 * c[3] + 8 * c[4] + 64 * c[0]  # 32 B ray in + 32 B record out
+        return {"iterations": round(c[1] / n, 3), "brick_visits": round(c[2] / n, 3),
+                "dda_voxels": round(c[3] / n, 3), "hit_rate": round(c[4] / n, 4), "bytes_per_ray": round(b / n, 2)}
+
+    if which in ("both", "random"):
+        g = torch.Generator(device="cuda").manual_seed(1)
+        n = 2_000_000
+        rr = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+        rr[:, 0:3] = torch.rand((n, 3), generator=g, device="cuda") * N
+        rr[:, 3] = 0.0
+        rr[:, 4:7] = torch.randn((n, 3), generator=g, device="cuda")
+        rr[:, 7] = 3.0e38
+        rr = rr.view(torch.uint8).view(-1)
+        h_rand = torch.empty_like(rr)
+        ms_rand = timed(torch, stream, lambda: scene.trace(rr, h_rand, stream=stream), steps, 2)
+        out.update({"random_ms": round(ms_rand, 4), "random_mrays": round(n / ms_rand / 1e3, 1),
+                    "random_per_ray": per_ray(rr, n)})
+    if which in ("both", "bounce"):
+        primary = vx.camera_rays(cam, W, H, 1e30)
+        bounce = scene.bounce_rays(primary, scene.trace(primary), seed=7)
+        h_bounce = torch.empty_like(bounce)
+        ms_bounce = timed(torch, stream, lambda: scene.trace(bounce, h_bounce, stream=stream), steps, 2)
+        out.update({"bounce_ms": round(ms_bounce, 4), "bounce_rays": W * H,
+                    "bounce_mrays": round(W * H / ms_bounce / 1e3, 1), "bounce_per_ray": per_ray(bounce, W * H)})
+    return out
 
 
 def urban_leg(vx, orc, torch, ctx, stream, steps: int, check: bool):
@@ -443,6 +459,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-fwdbwd", action="store_true", help="skip the differentiable albedo leg")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent-ray leg")
+    ap.add_argument("--incoherent-set", choices=("both", "random", "bounce"), default="both",
+                    help="which incoherent ray set to time (one set alone for per-kernel profiles)")
     ap.add_argument("--no-rows", action="store_true", help="skip the ingest/shading/LOD leg")
     ap.add_argument("--urban", action="store_true", help="add the 1024^3 urban leg (primary + bounce rays)")
     ap.add_argument("--vixen-json", default=None, help="also write the headline frames in VIXEN's results schema")
@@ -526,7 +544,8 @@ def main() -> int:
     got = vx.hits_to_numpy(hits)
     if not args.no_incoherent:
         log("[bench] incoherent-ray leg...")
-        result["incoherent"] = incoherent_leg(vx, torch, scene, cam, W, H, N, stream, max(5, args.steps // 5))
+        result["incoherent"] = incoherent_leg(vx, torch, scene, cam, W, H, N, stream, max(5, args.steps // 5),
+                                              args.incoherent_set)
         log(f"[bench] incoherent: {result['incoherent']}")
     if not args.no_fwdbwd:
         log("[bench] fwd+bwd leg...")

@@ -121,7 +121,20 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
     if ((threadIdx.x & 63u) == 0u) atomicAdd(dst, x);
 }
 
-template <bool COUNT, bool STAGED, bool LOD = false>
+// Rays of one 64-ray block (rays 64b .. 64b+63 below n, one per lane) that all
+// start at one origin (bit-equal to the block's first ray's): a pinhole
+// camera's rays. Such a block is traced one ray per lane in lockstep, leaves
+// visited at once; any other block postpones leaf visits (trace_ray's POSTPONE)
+// or, when SPLIT, is left to k_trace_refill.
+__device__ __forceinline__ bool block_shares_origin(const float o[3])
+{
+    const bool same = f2u(o[0]) == __builtin_amdgcn_readfirstlane(f2u(o[0])) &&
+                      f2u(o[1]) == __builtin_amdgcn_readfirstlane(f2u(o[1])) &&
+                      f2u(o[2]) == __builtin_amdgcn_readfirstlane(f2u(o[2]));
+    return __ballot(!same) == 0ull;
+}
+
+template <bool COUNT, bool STAGED, bool LOD = false, bool SPLIT = false>
 __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __restrict__ rays, uint64_t n,
                                                   uint32_t per_xcd, vx_hit* __restrict__ hits,
                                                   unsigned long long* __restrict__ counters)
@@ -130,15 +143,18 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
     const uint64_t i = (uint64_t)xcd_chunk(per_xcd) * kBlock + threadIdx.x;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     if (i < n) {
-        float o[3], d[3], tmin, tmax;
-        load_ray(rays, i, o, d, tmin, tmax);
+        // a wave holds one aligned 64-ray block; its origins decide how it is traced
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0];
+        const float o[3] = {a.x, a.y, a.z};
+        const float tmin = a.w;
+        const bool shared = block_shares_origin(o);
+        if (SPLIT && !shared) return; // wave-uniform: k_trace_refill traces this block
+        const float4 b = rp[1];
+        const float d[3] = {b.x, b.y, b.z};
+        const float tmax = b.w;
         vx_hit h;
-        // rays of one origin (a pinhole camera's) reach leaves together: visit them at once;
-        // otherwise postpone leaf visits until most of the wave waits at one
-        const bool same = f2u(o[0]) == __builtin_amdgcn_readfirstlane(f2u(o[0])) &&
-                          f2u(o[1]) == __builtin_amdgcn_readfirstlane(f2u(o[1])) &&
-                          f2u(o[2]) == __builtin_amdgcn_readfirstlane(f2u(o[2]));
-        if (__ballot(!same) == 0ull) trace_ray<COUNT, STAGED, false, LOD>(s, o, d, tmin, tmax, h, cnt);
+        if (shared) trace_ray<COUNT, STAGED, false, LOD>(s, o, d, tmin, tmax, h, cnt);
         else trace_ray<COUNT, STAGED, true, LOD>(s, o, d, tmin, tmax, h, cnt);
         store_hit(hits, i, h);
     }
@@ -148,6 +164,125 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
         wave_add_u64(&counters[2], cnt.leaves);
         wave_add_u64(&counters[3], cnt.voxels);
         wave_add_u64(&counters[4], cnt.hits);
+    }
+}
+
+// vx_trace's second launch: the 64-ray blocks whose rays do not share one
+// origin (random, bounce or other incoherent rays), traced with lane refill.
+// One ray per lane for a whole block wastes most of the wave on incoherent
+// rays: rays of a block end after very different numbers of iterations, so
+// few lanes stay active. Here a resident grid of waves keeps 64 rays in
+// flight per wave: whenever at least refill_min lanes are idle (between ESVO
+// iterations) they take the next rays of the wave's range (ranked by mbcnt).
+// The rays are split into one contiguous range per wave, of equal length (no
+// atomics or device state, so the launch stays capturable and re-entrant);
+// waves are numbered so each XCD takes a contiguous band of ranges (workgroup
+// i runs on XCD i % 8). A wave walks its range block by block (aligned
+// 64-ray blocks, classified exactly as k_trace classifies them) and skips the
+// blocks that share an origin, which k_trace<..., SPLIT> traces. Leaf visits
+// wait until 3/4 of the live lanes wait at one (trace_ray's POSTPONE). Each
+// lane executes exactly the reference's iteration sequence, so records equal
+// k_trace's bit for bit.
+template <bool STAGED, bool LOD>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 7)))
+    k_trace_refill(SceneView s, const vx_ray* __restrict__ rays, uint64_t n, uint64_t range, uint32_t refill_min,
+                   vx_hit* __restrict__ hits)
+{
+    stage_prefix<STAGED>(s);
+    constexpr uint32_t wpg = kBlock / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    // gridDim.x is a multiple of kXcds: XCD x owns waves [x W/8, (x+1) W/8)
+    const uint64_t v = ((uint64_t)(blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds) * wpg +
+                       (threadIdx.x >> 6);
+    uint64_t pos = min(v * range, n);            // next ray of the range not yet classified
+    const uint64_t last = min(pos + range, n);   // end of the range
+    uint64_t cur = pos, end = pos;               // rays of the current block not yet handed out
+    bool drained = false;
+
+    enum : int { IDLE = 0, ESVO = 1, LEAF = 2 };
+    int mode = IDLE;
+    uint64_t ray_id = 0;
+    RayState r;
+    vx_hit h;
+    TraceCounts cnt{0u, 0u, 0u, 0u};
+    for (;;) {
+        uint64_t idle = __ballot(mode == IDLE);
+        if (!drained && (uint32_t)__popcll(idle) >= refill_min) {
+            while (idle != 0ull) {
+                while (cur == end) { // the next block of the range that k_trace leaves to us
+                    if (pos >= last) {
+                        drained = true;
+                        break;
+                    }
+                    const uint64_t b = pos >> 6;
+                    const uint64_t stop = min(min((b + 1u) << 6, n), last);
+                    const uint64_t ri = (b << 6) + lane;
+                    bool shared = true;
+                    if (ri < n) {
+                        const float4 a = reinterpret_cast<const float4*>(rays + ri)[0]; // origin and tmin
+                        const float o[3] = {a.x, a.y, a.z};
+                        shared = block_shares_origin(o);
+                    }
+                    if (__ballot(!shared) == 0ull) { // every lane of the wave agrees
+                        pos = stop;
+                        continue;
+                    }
+                    cur = pos;
+                    end = stop;
+                    pos = stop;
+                }
+                if (drained) break;
+                const uint64_t take = min((uint64_t)__popcll(idle), end - cur);
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (mode == IDLE && rank < take) {
+                    ray_id = cur + rank;
+                    float o[3], d[3], tmin, tmax;
+                    load_ray(rays, ray_id, o, d, tmin, tmax);
+                    if (init_ray(s, o, d, tmin, tmax, r)) {
+                        rec_load(s, h);
+                        store_hit(hits, ray_id, h);
+                    } else {
+                        mode = ESVO;
+                    }
+                }
+                cur += take;
+                idle = __ballot(mode == IDLE);
+            }
+        }
+        const uint64_t live = __ballot(mode != IDLE);
+        if (live == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        bool done = false;
+        if (mode == ESVO) {
+            LeafRef lf;
+            const int st = esvo_step<false, STAGED, false, LOD>(s, r, cnt, lf);
+            if (st == STEP_DONE) {
+                done = true;
+            } else if (st == STEP_LEAF) {
+                *lane_word(s, s.aux_base, 11) = lf.brick;
+                *lane_word(s, s.aux_base, 12) = lf.lidx;
+                mode = LEAF;
+            }
+        }
+        const uint32_t waiting = (uint32_t)__popcll(__ballot(mode == LEAF));
+        if (waiting * 4u >= (uint32_t)__popcll(live) * 3u && mode == LEAF) {
+            int bc[3];
+            brick_cell(s, r, bc);
+            if (visit_leaf<false>(s, r, *lane_word(s, s.aux_base, 11), bc, *lane_word(s, s.aux_base, 12), cnt))
+                done = true;
+            else if (resume_after_leaf(s, r))
+                done = true;
+            else
+                mode = ESVO;
+        }
+        if (done) {
+            rec_load(s, h);
+            store_hit(hits, ray_id, h);
+            mode = IDLE;
+        }
     }
 }
 
@@ -639,6 +774,46 @@ static SceneView launch_view(const vx_scene* s, uint32_t& lds, bool parks)
     return v;
 }
 
+// Idle lanes that trigger a refill in k_trace_refill (measured on the 1080p
+// cave: 16 beats 8, 24 and 32 on random and bounce rays).
+constexpr uint32_t kRefillMin = 16;
+
+// vx_trace as two launches on one stream: k_trace<..., SPLIT> traces the 64-ray
+// blocks that share an origin, k_trace_refill the others. The refill grid is
+// as many workgroups as are resident at once (a multiple of the XCD count),
+// each wave taking an equal contiguous range of at least 64 rays.
+template <bool LOD>
+static vx_status launch_trace_split(const vx_scene* s, const vx_lod* lod, const vx_ray* rays, uint64_t n,
+                                    vx_hit* hits, const dim3& grid, uint32_t per, hipStream_t st)
+{
+    // k_trace<..., SPLIT> never parks a leaf: its lane columns are two words shorter
+    uint32_t ldsk = 0, lds = 0;
+    SceneView vk = launch_view(s, ldsk, false), v = launch_view(s, lds, true);
+    if (LOD) {
+        vk.lod_orig = v.lod_orig = lod->ray_orig_size;
+        vk.lod_dir = v.lod_dir = lod->ray_dir_size;
+    }
+    const bool staged = v.lds_count > 0;
+    const dim3 block(kBlock);
+    if (vk.lds_count > 0) k_trace<false, true, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
+    else k_trace<false, false, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
+    VX_HIP(hipGetLastError());
+    int per_cu = 0;
+    if (staged) VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<true, LOD>, kBlock, lds));
+    else VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<false, LOD>, kBlock, lds));
+    if (per_cu < 1) return fail(VX_ERR_UNSUPPORTED, "vx_trace: the refill kernel does not fit one workgroup per CU");
+    const uint64_t resident = (uint64_t)per_cu * (uint64_t)s->cu_count * (kBlock / 64);
+    const uint64_t range = std::max<uint64_t>(64, (n + resident - 1) / resident);
+    const uint64_t waves = (n + range - 1) / range;
+    uint64_t groups = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    groups = (groups + kXcds - 1) / kXcds * kXcds;
+    const dim3 rgrid(static_cast<unsigned>(groups));
+    if (staged) k_trace_refill<true, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, kRefillMin, hits);
+    else k_trace_refill<false, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, kRefillMin, hits);
+    VX_HIP(hipGetLastError());
+    return VX_OK;
+}
+
 static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hit* hits, uint64_t* counters,
                               hipStream_t st, const vx_lod* lod = nullptr)
 {
@@ -650,21 +825,17 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     if (per * kXcds > 0x7FFFFFFFull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     const dim3 grid(static_cast<unsigned>(per * kXcds)), block(kBlock);
     uint32_t lds = 0;
-    SceneView v = launch_view(s, lds, true); // k_trace postpones leaf visits
+    const SceneView v = launch_view(s, lds, true); // the counting k_trace postpones leaf visits
     const bool staged = v.lds_count > 0;
     if (lod && lod->ray_dir_size > 0.0f) { // LODParameters::isEnabled
-        v.lod_orig = lod->ray_orig_size;
-        v.lod_dir = lod->ray_dir_size;
-        if (staged) k_trace<false, true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
-        else k_trace<false, false, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        return launch_trace_split<true>(s, lod, rays, n, hits, grid, static_cast<uint32_t>(per), st);
     } else if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
         auto* c = reinterpret_cast<unsigned long long*>(counters);
         if (staged) k_trace<true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
         else k_trace<true, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
     } else {
-        if (staged) k_trace<false, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
-        else k_trace<false, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, nullptr);
+        return launch_trace_split<false>(s, nullptr, rays, n, hits, grid, static_cast<uint32_t>(per), st);
     }
     VX_HIP(hipGetLastError());
     return VX_OK;
