@@ -365,6 +365,24 @@ vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t 
                              const float light[3], const float* grad_image, const uint32_t* hit_row,
                              const float* hit_shade, float* grad_albedo, uint32_t mode, vx_stream stream);
 
+/* No counterpart in the reference: vx_albedo_backward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training or
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, results
+ * and modes; bit-identical gradients. It calls no hipMalloc/hipFree: the
+ * deterministic mode's workspace (28 bytes per pixel plus the sort's temporary
+ * storage) lives in a fixed 48 MiB device array that the library holds on
+ * every device it runs on until the process ends, so width * height is at most
+ * VX_NOALLOC_MAX_PIXELS (larger images: VX_ERR_INVALID_ARG). The scene's voxel
+ * row table must exist already: vx_scene_voxel_count or a vx_albedo_forward on
+ * the scene builds it (that call allocates); otherwise VX_ERR_INVALID_ARG.
+ * A context must have been created on the scene's device. Every call on one
+ * device shares the one workspace, so calls that overlap on a device (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define VX_NOALLOC_MAX_PIXELS (1u << 20)
+vx_status vx_albedo_backward_noalloc(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
+                                     const float light[3], const float* grad_image, const uint32_t* hit_row,
+                                     const float* hit_shade, float* grad_albedo, uint32_t mode, vx_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

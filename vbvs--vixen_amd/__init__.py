@@ -190,6 +190,7 @@ def _load() -> C.CDLL:
         "vx_scene_voxel_count": ([vp, C.POINTER(u64)], st),
         "vx_albedo_forward": ([vp, C.POINTER(VxCamera), u32, u32, vp, vp, vp, vp, vp, vp], st),
         "vx_albedo_backward": ([vp, C.POINTER(VxCamera), u32, u32, vp, vp, vp, vp, vp, u32, vp], st),
+        "vx_albedo_backward_noalloc": ([vp, C.POINTER(VxCamera), u32, u32, vp, vp, vp, vp, vp, u32, vp], st),
         "vx_gen_cornell": ([u32, u64, vp, C.POINTER(VxCamera)], st),
         "vx_gen_cave": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
         "vx_gen_urban": ([u32, u64, C.c_double, vp, C.POINTER(VxCamera)], st),
@@ -213,7 +214,8 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_graph_destroy", "vx_scene_voxel_count", "vx_albedo_forward", "vx_albedo_backward", "vx_bounce_rays",
             "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
             "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
-            "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device")
+            "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device",
+            "vx_albedo_backward_noalloc")
 
 
 def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:

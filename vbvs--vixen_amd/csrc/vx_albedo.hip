@@ -30,7 +30,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
+#include <string>
 
 #include "vx_internal.hpp"
 
@@ -317,6 +319,29 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* 
 
 __global__ void k_clear_long_count(uint32_t* counts) { counts[1] = 0u; }
 
+// Device buffers of the deterministic reduction for up to cap_px pixels.
+struct DetWorkspace {
+    uint32_t *keys, *keys_sorted, *heads;
+    uint32_t* n_runs; // [0] runs, [1] long runs
+    Rgb* vals;        // contributions, by pixel
+    uint32_t *pix, *pix_sorted;
+    void* tmp;        // sort and run-selection temporary storage
+    size_t cap_px, cap_tmp;
+};
+
+// Workspace of vx_albedo_backward_noalloc: one array per device, kept until the
+// process ends, shared by every call on that device (calls that overlap on one
+// device race on it). Host code never uses its name as an address: its device
+// address comes from hipGetSymbolAddress, once per device, at vx_ctx_create.
+constexpr size_t kScratchBytes = size_t{48} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static hipError_t g_scratch_err[kMaxDevices] = {};
+static bool g_scratch_done[kMaxDevices] = {};
+
 } // namespace vx
 
 // Per-scene state of the albedo path (built on first use).
@@ -324,12 +349,7 @@ struct vx_albedo_state {
     uint32_t* base = nullptr; // per-brick row base
     uint64_t voxels = 0;
     // sorted-reduction workspace, grown on demand
-    size_t cap_px = 0, cap_tmp = 0;
-    uint32_t *keys = nullptr, *keys_sorted = nullptr, *heads = nullptr;
-    uint32_t* n_runs = nullptr; // [0] runs, [1] long runs
-    vx::Rgb* vals = nullptr;                         // contributions, by pixel
-    uint32_t *pix = nullptr, *pix_sorted = nullptr; // pixel index, sorted with the keys
-    void* tmp = nullptr;
+    vx::DetWorkspace ws{};
 };
 
 namespace vx {
@@ -401,24 +421,68 @@ static SceneView alb_view(const vx_scene* s, uint32_t& lds)
 
 static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
 {
-    if (px > a->cap_px) {
-        for (void* p : {(void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs, (void*)a->vals,
-                        (void*)a->pix, (void*)a->pix_sorted})
+    DetWorkspace& w = a->ws;
+    if (px > w.cap_px) {
+        for (void* p : {(void*)w.keys, (void*)w.keys_sorted, (void*)w.heads, (void*)w.n_runs, (void*)w.vals,
+                        (void*)w.pix, (void*)w.pix_sorted})
             (void)hipFree(p);
-        a->cap_px = 0;
-        if (hipMalloc(&a->keys, px * 4) != hipSuccess || hipMalloc(&a->keys_sorted, px * 4) != hipSuccess ||
-            hipMalloc(&a->heads, px * 4) != hipSuccess || hipMalloc(&a->n_runs, 8) != hipSuccess ||
-            hipMalloc(&a->vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&a->pix, px * 4) != hipSuccess ||
-            hipMalloc(&a->pix_sorted, px * 4) != hipSuccess)
+        w.keys = w.keys_sorted = w.heads = w.n_runs = w.pix = w.pix_sorted = nullptr;
+        w.vals = nullptr;
+        w.cap_px = 0;
+        if (hipMalloc(&w.keys, px * 4) != hipSuccess || hipMalloc(&w.keys_sorted, px * 4) != hipSuccess ||
+            hipMalloc(&w.heads, px * 4) != hipSuccess || hipMalloc(&w.n_runs, 8) != hipSuccess ||
+            hipMalloc(&w.vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&w.pix, px * 4) != hipSuccess ||
+            hipMalloc(&w.pix_sorted, px * 4) != hipSuccess)
             return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
-        a->cap_px = px;
+        w.cap_px = px;
     }
-    if (tmp > a->cap_tmp) {
-        (void)hipFree(a->tmp);
-        a->cap_tmp = 0;
-        if (hipMalloc(&a->tmp, tmp) != hipSuccess) return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
-        a->cap_tmp = tmp;
+    if (tmp > w.cap_tmp) {
+        (void)hipFree(w.tmp);
+        w.tmp = nullptr;
+        w.cap_tmp = 0;
+        if (hipMalloc(&w.tmp, tmp) != hipSuccess) return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
+        w.cap_tmp = tmp;
     }
+    return VX_OK;
+}
+
+// The deterministic reduction's workspace for px pixels carved from the
+// device's scratch array: 256-byte aligned slices, the temporary storage last.
+// Fails when px pixels and tmp bytes do not fit.
+static vx_status scratch_workspace(int device, size_t px, size_t tmp, DetWorkspace& w)
+{
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (device < 0 || device >= kMaxDevices || !g_scratch_done[device])
+            return fail(VX_ERR_INVALID_ARG, "vx_albedo_backward_noalloc: no context was created on device " +
+                                                std::to_string(device));
+        if (g_scratch_err[device] != hipSuccess)
+            return fail(VX_ERR_HIP, std::string("vx_albedo_backward_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                                        hipGetErrorString(g_scratch_err[device]));
+        base = g_scratch_ptr[device];
+    }
+    auto* p = static_cast<unsigned char*>(base);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void* q = p + off;
+        off += (bytes + 255) & ~size_t{255};
+        return q;
+    };
+    w.keys = static_cast<uint32_t*>(take(px * 4));
+    w.keys_sorted = static_cast<uint32_t*>(take(px * 4));
+    w.heads = static_cast<uint32_t*>(take(px * 4));
+    w.n_runs = static_cast<uint32_t*>(take(8));
+    w.vals = static_cast<Rgb*>(take(px * sizeof(Rgb)));
+    w.pix = static_cast<uint32_t*>(take(px * 4));
+    w.pix_sorted = static_cast<uint32_t*>(take(px * 4));
+    w.tmp = take(tmp);
+    w.cap_px = px;
+    w.cap_tmp = tmp;
+    if (off > kScratchBytes)
+        return fail(VX_ERR_INVALID_ARG, "vx_albedo_backward_noalloc: " + std::to_string(px) + " pixels need " +
+                                            std::to_string(off) + " bytes of workspace, the scratch array holds " +
+                                            std::to_string(kScratchBytes));
     return VX_OK;
 }
 
@@ -427,8 +491,9 @@ static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
 void vx_albedo_state_free(vx_albedo_state* a)
 {
     if (!a) return;
-    for (void* p : {(void*)a->base, (void*)a->keys, (void*)a->keys_sorted, (void*)a->heads, (void*)a->n_runs,
-                    (void*)a->vals, (void*)a->pix, (void*)a->pix_sorted, a->tmp})
+    const vx::DetWorkspace& w = a->ws;
+    for (void* p : {(void*)a->base, (void*)w.keys, (void*)w.keys_sorted, (void*)w.heads, (void*)w.n_runs,
+                    (void*)w.vals, (void*)w.pix, (void*)w.pix_sorted, w.tmp})
         (void)hipFree(p);
     delete a;
 }
@@ -468,80 +533,133 @@ vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint3
     return VX_OK;
 }
 
-vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
-                             const float* grad_image, const uint32_t* hit_row, const float* hit_shade,
-                             float* grad_albedo, uint32_t mode, vx_stream stream)
+} // extern "C"
+
+namespace vx {
+
+// vx_albedo_backward's body. The deterministic reduction's workspace comes from
+// the scene (grown with hipMalloc) unless noalloc, when it is carved from the
+// device's scratch array and the call allocates nothing.
+static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
+                                 const float* grad_image, const uint32_t* hit_row, const float* hit_shade,
+                                 float* grad_albedo, uint32_t mode, hipStream_t st, bool noalloc)
 {
+    const char* where = noalloc ? "vx_albedo_backward_noalloc" : "vx_albedo_backward";
     if (!s || !cam || !light || !grad_image || !grad_albedo || (!hit_row) != (!hit_shade) || mode > 1)
-        return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: bad arguments");
+        return fail(VX_ERR_INVALID_ARG, std::string(where) + ": bad arguments");
     if (w == 0 || h == 0) return VX_OK;
-    if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: image too large");
+    if (w > 65536 || h > 65536) return fail(VX_ERR_INVALID_ARG, std::string(where) + ": image too large");
+    const size_t px = (size_t)w * h;
+    if (noalloc && px > VX_NOALLOC_MAX_PIXELS)
+        return fail(VX_ERR_INVALID_ARG, std::string(where) + ": " + std::to_string(w) + "x" + std::to_string(h) +
+                                            " is " + std::to_string(px) + " pixels, more than the " +
+                                            std::to_string(VX_NOALLOC_MAX_PIXELS) + " its workspace holds");
     VX_DEVICE_GUARD(s->device);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    vx_status status;
-    vx_albedo_state* a = vx::albedo_state(s, st, status);
-    if (!a) return status;
+    vx_albedo_state* a = s->albedo;
+    if (!a && noalloc)
+        return fail(VX_ERR_INVALID_ARG, std::string(where) + ": the scene has no voxel row table yet (it is built, "
+                                                             "with an allocation, by vx_scene_voxel_count or the first "
+                                                             "vx_albedo_forward)");
+    vx_status status = VX_OK;
+    if (!a && !(a = albedo_state(s, st, status))) return status;
     uint32_t lds = 0;
-    const vx::SceneView v = vx::alb_view(s, lds);
-    const vx::AlbCam c = vx::make_alb_cam(cam, light);
+    const SceneView v = alb_view(s, lds);
+    const AlbCam c = make_alb_cam(cam, light);
     const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
     const bool recompute = hit_row == nullptr;
-    const dim3 grid(tiles), block(vx::kAlbBlock);
-    const size_t px = (size_t)w * h;
+    const dim3 grid(tiles), block(kAlbBlock);
     if (mode == VX_GRAD_ATOMIC) {
         if (recompute)
-            vx::k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr,
-                                                                       nullptr, grad_albedo, nullptr, nullptr, nullptr, 0u);
+            k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
+                                                                   grad_albedo, nullptr, nullptr, nullptr, 0u);
         else
-            vx::k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
-                                                                      hit_shade, grad_albedo, nullptr, nullptr, nullptr,
-                                                                      0u);
-        if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
+            k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
+                                                                  grad_albedo, nullptr, nullptr, nullptr, 0u);
+        if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
         return VX_OK;
     }
     // deterministic: sort (row, contribution) by row, select the run heads, sum each run in a fixed order
-    if (px > 0x7FFFFFFFu) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_backward: deterministic mode needs w*h < 2^31");
-    if (a->voxels >= 0xFFFFFFFFull) return vx::fail(VX_ERR_UNSUPPORTED, "vx_albedo_backward: too many voxels");
+    if (px > 0x7FFFFFFFu) return fail(VX_ERR_INVALID_ARG, std::string(where) + ": deterministic mode needs w*h < 2^31");
+    if (a->voxels >= 0xFFFFFFFFull) return fail(VX_ERR_UNSUPPORTED, std::string(where) + ": too many voxels");
     // misses get key V (the voxel count) and sort last; the sort looks at the key bits V needs
     const uint32_t miss_key = (uint32_t)a->voxels;
     int end_bit = 1;
     while (end_bit < 32 && (miss_key >> end_bit) != 0u) ++end_bit;
     size_t sort_bytes = 0, red_bytes = 0;
     const hipcub::CountingInputIterator<uint32_t> index(0u);
-    if (rocprim::radix_sort_pairs<vx::PairSortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                      (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)px, 0u,
-                                                      (unsigned)end_bit, st) != hipSuccess ||
+    if (rocprim::radix_sort_pairs<PairSortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)px, 0u,
+                                                  (unsigned)end_bit, st) != hipSuccess ||
         hipcub::DeviceSelect::If(nullptr, red_bytes, index, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px,
-                                 vx::RunHead{nullptr}, st) != hipSuccess)
-        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: workspace query failed");
-    if (vx_status g = vx::grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
+                                 RunHead{nullptr}, st) != hipSuccess)
+        return fail(VX_ERR_HIP, std::string(where) + ": workspace query failed");
+    DetWorkspace ws{};
+    if (noalloc) {
+        if (vx_status g = scratch_workspace(s->device, px, std::max(sort_bytes, red_bytes), ws); g != VX_OK) return g;
+    } else {
+        if (vx_status g = grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
+        ws = a->ws;
+    }
     if (recompute)
-        vx::k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
-                                                                    nullptr, a->keys, a->vals, a->pix, miss_key);
+        k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
+                                                                nullptr, ws.keys, ws.vals, ws.pix, miss_key);
     else
-        vx::k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row,
-                                                                   hit_shade, nullptr, a->keys, a->vals, a->pix,
-                                                                   miss_key);
-    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: launch failed");
-    size_t tb = a->cap_tmp;
-    if (rocprim::radix_sort_pairs<vx::PairSortConfig>(a->tmp, tb, a->keys, a->keys_sorted, a->pix, a->pix_sorted,
-                                                      (uint32_t)px, 0u, (unsigned)end_bit, st) != hipSuccess)
-        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: sort failed");
-    tb = a->cap_tmp;
-    if (hipcub::DeviceSelect::If(a->tmp, tb, index, a->heads, a->n_runs, (int)px, vx::RunHead{a->keys_sorted}, st) !=
+        k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
+                                                               nullptr, ws.keys, ws.vals, ws.pix, miss_key);
+    if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
+    size_t tb = ws.cap_tmp;
+    if (rocprim::radix_sort_pairs<PairSortConfig>(ws.tmp, tb, ws.keys, ws.keys_sorted, ws.pix, ws.pix_sorted,
+                                                  (uint32_t)px, 0u, (unsigned)end_bit, st) != hipSuccess)
+        return fail(VX_ERR_HIP, std::string(where) + ": sort failed");
+    tb = ws.cap_tmp;
+    if (hipcub::DeviceSelect::If(ws.tmp, tb, index, ws.heads, ws.n_runs, (int)px, RunHead{ws.keys_sorted}, st) !=
         hipSuccess)
-        return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run selection failed");
+        return fail(VX_ERR_HIP, std::string(where) + ": run selection failed");
     // resident groups walk all runs (their count stays on the device); the keys
     // buffer, consumed by the sort, holds the list of long runs
     const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->cu_count * 8u,
-                                                                               (px + vx::kAlbBlock - 1) / vx::kAlbBlock));
-    vx::k_clear_long_count<<<1, 1, 0, st>>>(a->n_runs);
-    vx::k_reduce_short_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals, a->pix_sorted, a->heads, a->n_runs,
-                                                            (uint32_t)px, miss_key, grad_albedo, a->keys);
-    vx::k_reduce_long_runs<<<dim3(groups), block, 0, st>>>(a->keys_sorted, a->vals, a->pix_sorted, a->heads, a->n_runs,
-                                                           (uint32_t)px, a->keys, grad_albedo);
-    if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_backward: run reduction failed");
+                                                                               (px + kAlbBlock - 1) / kAlbBlock));
+    k_clear_long_count<<<1, 1, 0, st>>>(ws.n_runs);
+    k_reduce_short_runs<<<dim3(groups), block, 0, st>>>(ws.keys_sorted, ws.vals, ws.pix_sorted, ws.heads, ws.n_runs,
+                                                        (uint32_t)px, miss_key, grad_albedo, ws.keys);
+    k_reduce_long_runs<<<dim3(groups), block, 0, st>>>(ws.keys_sorted, ws.vals, ws.pix_sorted, ws.heads, ws.n_runs,
+                                                       (uint32_t)px, ws.keys, grad_albedo);
+    if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": run reduction failed");
     return VX_OK;
+}
+
+// At vx_ctx_create, with the device current: the device address of its scratch
+// array (each device holds its own copy), once per device.
+void scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
+} // namespace vx
+
+extern "C" {
+
+vx_status vx_albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
+                             const float* grad_image, const uint32_t* hit_row, const float* hit_shade,
+                             float* grad_albedo, uint32_t mode, vx_stream stream)
+{
+    return vx::albedo_backward(s, cam, w, h, light, grad_image, hit_row, hit_shade, grad_albedo, mode,
+                               static_cast<hipStream_t>(stream), false);
+}
+
+vx_status vx_albedo_backward_noalloc(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, const float light[3],
+                                     const float* grad_image, const uint32_t* hit_row, const float* hit_shade,
+                                     float* grad_albedo, uint32_t mode, vx_stream stream)
+{
+    return vx::albedo_backward(s, cam, w, h, light, grad_image, hit_row, hit_shade, grad_albedo, mode,
+                               static_cast<hipStream_t>(stream), true);
 }
 
 } // extern "C"

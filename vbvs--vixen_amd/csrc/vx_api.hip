@@ -861,6 +861,7 @@ vx_status vx_ctx_create(int device, vx_ctx** out)
         return vx::fail(VX_ERR_INVALID_ARG, "vx_ctx_create: device " + std::to_string(device) + " not in [0, " +
                                                 std::to_string(count) + ")");
     if (hipSetDevice(device) != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_ctx_create: hipSetDevice failed");
+    vx::scratch_init(device); // vx_albedo_backward_noalloc's workspace on this device
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
     auto* c = new (std::nothrow) vx_ctx{device, cus};

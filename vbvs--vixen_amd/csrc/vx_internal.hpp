@@ -14,6 +14,10 @@
 
 struct vx_albedo_state;
 void vx_albedo_state_free(vx_albedo_state* a);
+namespace vx {
+// Looks up the device address of the no-allocation workspace on `device` (current).
+void scratch_init(int device);
+} // namespace vx
 
 struct vx_ctx {
     int device;
