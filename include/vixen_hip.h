@@ -25,7 +25,10 @@
  *     in HBM); scene creation takes HOST pointers and copies;
  *   - vx_trace, vx_trace_lod, vx_render and vx_camera_rays only launch kernels
  *     (no allocation, memset or synchronisation), so a caller may record them
- *     into a HIP graph with stream capture and replay the frame.
+ *     into a HIP graph with stream capture and replay the frame. Such a graph
+ *     holds the scene's device pointers but no reference to the scene: keep
+ *     the scene (take one with vx_scene_retain) and any buffers it borrows
+ *     alive until the graph is destroyed.
  */
 #ifndef VIXEN_HIP_H
 #define VIXEN_HIP_H

@@ -154,3 +154,25 @@ def test_builder_rejects_bad_sizes(vx):
     assert vx.LIB.vx_build_from_dense(g.ctypes.data, 8, 1, 1, C.byref(out)) == 1
     assert b"power of two" in vx.LIB.vx_last_error()
     assert vx.LIB.vx_build_from_dense(None, 16, 1, 1, C.byref(out)) == 1
+
+
+def test_python_mirror_checks_buffers_before_any_launch(vx):
+    # device buffers must be contiguous CUDA tensors on the scene's device, large enough
+    import torch
+    with pytest.raises(ValueError, match="cuda:0"):
+        vx._dev_buf(torch.zeros(64, dtype=torch.uint8), "hits", 0, 32)
+    with pytest.raises(ValueError, match="torch tensor"):
+        vx._dev_buf(np.zeros(64, np.uint8), "hits", 0, 32)
+
+
+def test_batched_ray_offsets_are_checked(vx):
+    # one entry per scene plus one, non-decreasing, ending inside the rays (the C side reads off[n_scenes]
+    # and trusts the rest)
+    got = vx._ray_offsets([0, 10, 10, 25], 3, 25)
+    assert list(got) == [0, 10, 10, 25]
+    with pytest.raises(ValueError, match="entries"):
+        vx._ray_offsets([0, 10, 25], 3, 25)            # one entry short
+    with pytest.raises(ValueError, match="non-decreasing"):
+        vx._ray_offsets([0, 10, 5, 25], 3, 25)
+    with pytest.raises(ValueError, match="the rays hold"):
+        vx._ray_offsets([0, 10, 10, 26], 3, 25)        # past the last ray

@@ -60,3 +60,36 @@ def test_captured_frame_replays_bit_exact(vx, cave, gpu_ctx):
             assert np.array_equal(got.cpu().numpy(), want), f"replay {replay}: {name} differs from the eager launch"
     del graph
     scene.close()
+
+
+def test_captured_frame_keeps_its_scene(vx, cave, gpu_ctx):
+    # vx.CapturedFrame retains the scene: its creator closes it, the frame still replays bit-exact
+    import torch
+    grid, _, built = cave
+    scene = vx.Scene(gpu_ctx, built)
+    cam = vx.seeded_views(grid, 92, 1)[0]
+    ref_hits, _ = scene.render(cam, W, H)
+    torch.cuda.synchronize()
+    ref = ref_hits.cpu().numpy().copy()
+    out = torch.empty_like(ref_hits)
+
+    def record():
+        vx._check(vx.LIB.vx_render(scene._h, vx.C.byref(cam), W, H, 1e30, out.data_ptr(), None,
+                                   torch.cuda.current_stream().cuda_stream), "vx_render")
+        return out
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        record()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    frame = vx.CapturedFrame([scene], record, keep=(out,))
+    scene.close()  # the frame's reference keeps the device buffers alive
+    for _ in range(3):
+        out.fill_(0x5A)
+        torch.cuda.synchronize()
+        frame.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), ref)
+    frame.close()

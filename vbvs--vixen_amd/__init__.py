@@ -525,6 +525,52 @@ def _stream_ptr(stream) -> int | None:
     return int(s.cuda_stream) or None
 
 
+def _dev_buf(t, what: str, device: int, min_bytes: int = 0) -> int:
+    """The device address of t after checking it is a contiguous CUDA tensor on `device` holding at least
+    min_bytes bytes; ValueError otherwise (before anything is launched)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what} must be a torch tensor, got {type(t).__name__}")
+    if not t.is_cuda or t.device.index != device:
+        raise ValueError(f"{what} must live on cuda:{device}, it is on {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    nbytes = t.numel() * t.element_size()
+    if nbytes < min_bytes:
+        raise ValueError(f"{what} holds {nbytes} bytes, {min_bytes} needed")
+    return t.data_ptr()
+
+
+def _ray_count(rays, what: str, device: int) -> int:
+    _dev_buf(rays, what, device)
+    nbytes = rays.numel() * rays.element_size()
+    if nbytes % 32:
+        raise ValueError(f"{what} holds {nbytes} bytes, not a whole number of 32-byte vx_ray records")
+    return nbytes // 32
+
+
+def _ray_offsets(offsets, n_scenes: int, n_rays: int) -> C.Array:
+    """A batched call's ray_offsets as the host array the C ABI reads (n_scenes + 1 entries), after checking
+    it has that many entries, never decreases and ends within the n_rays rays; ValueError otherwise."""
+    o = np.ascontiguousarray(offsets, dtype=np.uint64).reshape(-1)
+    if len(o) != n_scenes + 1:
+        raise ValueError(f"ray_offsets needs {n_scenes + 1} entries (scenes + 1), got {len(o)}")
+    if np.any(o[1:] < o[:-1]):
+        raise ValueError("ray_offsets must be non-decreasing")
+    if int(o[-1]) > n_rays:
+        raise ValueError(f"ray_offsets ends at ray {int(o[-1])}, the rays hold {n_rays}")
+    return (C.c_uint64 * len(o))(*[int(x) for x in o])
+
+
+def _hits_for(hits, n: int, like, device: int, what: str = "hits"):
+    """The caller's hit buffer (checked to hold n records) or a new one."""
+    torch = _torch()
+    if hits is None:
+        return torch.empty(n * 32, dtype=torch.uint8, device=like.device)
+    _dev_buf(hits, what, device, 32 * n)
+    return hits
+
+
 class Context:
     """One device (replaces the Vulkan device/queue around ComputeDispatchNode)."""
 
@@ -561,10 +607,20 @@ class Scene:
     @classmethod
     def from_device(cls, ctx: Context, nodes, brick_data, config: VxOctreeConfig, bc1=None, normals=None) -> "Scene":
         """A scene over caller-owned device buffers (vx_scene_create_device), e.g. CUDA tensors holding the
-        descriptor pairs, brick words, BC1 blocks and DXT normal blocks; they are used in place, so they are
-        kept referenced by the scene."""
+        descriptor pairs, brick words, BC1 blocks and DXT normal blocks. They are used in place: the scene keeps
+        them referenced, and they must stay unchanged until close() (the kernels read them without range
+        checks; the tree was validated once, at creation). Each must be a contiguous CUDA tensor on the
+        context's device; bc1 and normals must hold 256 and 512 bytes per brick."""
         self = cls.__new__(cls)
         self.ctx, self.built = ctx, None
+        dev = ctx.device
+        _dev_buf(nodes, "nodes", dev)
+        _dev_buf(brick_data, "brick_data", dev)
+        brick_count = brick_data.numel() * brick_data.element_size() // 2048
+        if bc1 is not None:
+            _dev_buf(bc1, "bc1", dev, 256 * brick_count)
+        if normals is not None:
+            _dev_buf(normals, "normals", dev, 512 * brick_count)
         self._cfg = VxOctreeConfig()
         C.memmove(C.addressof(self._cfg), C.addressof(config), C.sizeof(config))
         self._keep = (nodes, brick_data, bc1, normals)
@@ -572,7 +628,7 @@ class Scene:
         desc.esvo_nodes = nodes.data_ptr()
         desc.node_count = nodes.numel() * nodes.element_size() // 8
         desc.brick_data = brick_data.data_ptr() if brick_data.numel() else None
-        desc.brick_count = brick_data.numel() * brick_data.element_size() // 2048
+        desc.brick_count = brick_count
         desc.bc1_colors = bc1.data_ptr() if bc1 is not None else None
         desc.dxt_normals = normals.data_ptr() if normals is not None else None
         desc.config = C.pointer(self._cfg)
@@ -592,30 +648,25 @@ class Scene:
         except Exception:
             pass
 
-    # rays/hits are torch uint8 or float32 CUDA tensors holding n * 32 bytes
+    # rays/hits are contiguous CUDA tensors (any dtype) on the scene's device holding n * 32 bytes
     def trace(self, rays, hits=None, stream=None):
-        torch = _torch()
-        n = rays.numel() * rays.element_size() // 32
-        if hits is None:
-            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        n = _ray_count(rays, "rays", self.ctx.device)
+        hits = _hits_for(hits, n, rays, self.ctx.device)
         _check(LIB.vx_trace(self._h, rays.data_ptr(), n, hits.data_ptr(), _stream_ptr(stream)), "vx_trace")
         return hits
 
     def trace_lod(self, rays, lod: VxLod, hits=None, stream=None):
         """castRayWithLOD over a batch; LOD hits carry VX_HIT_FLAG_LOD (bit 6)."""
-        torch = _torch()
-        n = rays.numel() * rays.element_size() // 32
-        if hits is None:
-            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        n = _ray_count(rays, "rays", self.ctx.device)
+        hits = _hits_for(hits, n, rays, self.ctx.device)
         _check(LIB.vx_trace_lod(self._h, rays.data_ptr(), n, C.byref(lod), hits.data_ptr(), _stream_ptr(stream)),
                "vx_trace_lod")
         return hits
 
     def trace_counters(self, rays, hits=None, stream=None):
         torch = _torch()
-        n = rays.numel() * rays.element_size() // 32
-        if hits is None:
-            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+        n = _ray_count(rays, "rays", self.ctx.device)
+        hits = _hits_for(hits, n, rays, self.ctx.device)
         counters = torch.zeros(8, dtype=torch.int64, device=rays.device)
         _check(LIB.vx_trace_counters(self._h, rays.data_ptr(), n, hits.data_ptr(), counters.data_ptr(),
                                      _stream_ptr(stream)), "vx_trace_counters")
@@ -634,7 +685,8 @@ class Scene:
     def bounce_rays(self, primary, hits, seed: int, stream=None):
         """Seeded cosine-weighted diffuse bounce rays for the primary hits (device buffers of n x 32 bytes)."""
         torch = _torch()
-        n = primary.numel() * primary.element_size() // 32
+        n = _ray_count(primary, "primary", self.ctx.device)
+        _dev_buf(hits, "hits", self.ctx.device, 32 * n)
         out = torch.empty(n * 32, dtype=torch.uint8, device=primary.device)
         _check(LIB.vx_bounce_rays(self._h, primary.data_ptr(), hits.data_ptr(), n, seed & 0xFFFFFFFF, out.data_ptr(),
                                   _stream_ptr(stream)), "vx_bounce_rays")
@@ -652,7 +704,10 @@ class Scene:
     def albedo_forward(self, cam: VxCamera, w: int, h: int, light, albedo, store_hits: bool = True, stream=None):
         """image [h,w,3] (+ per-pixel albedo row and shade when store_hits)."""
         torch = _torch()
-        assert albedo.dtype == torch.float32 and albedo.is_contiguous() and albedo.shape == (self.voxel_count(), 3)
+        _dev_buf(albedo, "albedo", self.ctx.device)
+        if albedo.dtype != torch.float32 or tuple(albedo.shape) != (self.voxel_count(), 3):
+            raise ValueError(f"albedo must be float32 [{self.voxel_count()}, 3], got {albedo.dtype} "
+                             f"{list(albedo.shape)}")
         lt = np.ascontiguousarray(light, dtype=np.float32)
         image = torch.empty((h, w, 3), dtype=torch.float32, device=albedo.device)
         rows = torch.empty(h * w, dtype=torch.int32, device=albedo.device) if store_hits else None
@@ -666,9 +721,20 @@ class Scene:
                         deterministic: bool = True, stream=None):
         """grad [V,3] += adjoint of albedo_forward for grad_image [h,w,3] (hits re-traced when rows is None)."""
         torch = _torch()
-        assert grad_image.dtype == torch.float32 and grad_image.is_contiguous() and grad_image.numel() == h * w * 3
+        dev = self.ctx.device
+        _dev_buf(grad_image, "grad_image", dev)
+        if grad_image.dtype != torch.float32 or grad_image.numel() != h * w * 3:
+            raise ValueError(f"grad_image must be float32 with {h * w * 3} elements")
+        if (rows is None) != (shade is None):
+            raise ValueError("rows and shade go together (both stored hits, or neither to re-trace)")
+        if rows is not None:
+            _dev_buf(rows, "rows", dev, 4 * w * h)
+            _dev_buf(shade, "shade", dev, 4 * w * h)
+        V = self.voxel_count()
         if grad is None:
-            grad = torch.zeros((self.voxel_count(), 3), dtype=torch.float32, device=grad_image.device)
+            grad = torch.zeros((V, 3), dtype=torch.float32, device=grad_image.device)
+        else:
+            _dev_buf(grad, "grad", dev, 12 * V)
         lt = np.ascontiguousarray(light, dtype=np.float32)
         _check(LIB.vx_albedo_backward(self._h, C.byref(cam), w, h, lt.ctypes.data, grad_image.data_ptr(),
                                       rows.data_ptr() if rows is not None else None,
@@ -705,41 +771,52 @@ class Batch:
         except Exception:
             pass
 
-    @staticmethod
-    def _offsets(offsets) -> C.Array:
-        o = np.ascontiguousarray(offsets, dtype=np.uint64)
-        return (C.c_uint64 * len(o))(*[int(x) for x in o])
+    def _args(self, rays, offsets, hits, need_hits: bool):
+        """Checks a batched call's buffers before anything is launched: ray_offsets has one entry per scene
+        plus one, never decreases, and its last entry fits both the rays and the hits."""
+        dev = self.ctx.device
+        offs = _ray_offsets(offsets, len(self.scenes), _ray_count(rays, "rays", dev))
+        if hits is None and need_hits:
+            raise ValueError("hits is required")
+        if hits is not None:
+            _dev_buf(hits, "hits", dev, 32 * int(offs[len(self.scenes)]))
+        return offs
 
     def trace(self, rays, offsets, hits=None, tuning=None, stream=None):
         torch = _torch()
-        n = rays.numel() * rays.element_size() // 32
         if hits is None:
-            hits = torch.empty(n * 32, dtype=torch.uint8, device=rays.device)
+            hits = torch.empty(rays.numel() * rays.element_size(), dtype=torch.uint8, device=rays.device)
+        offs = self._args(rays, offsets, hits, True)
         t = _tuning(tuning)
-        _check(LIB.vx_trace_batched(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+        _check(LIB.vx_trace_batched(self._h, rays.data_ptr(), offs, hits.data_ptr(),
                                     C.byref(t) if t is not None else None, _stream_ptr(stream)), "vx_trace_batched")
         return hits
 
     def autotune(self, rays, offsets, hits, stream=None):
+        offs = self._args(rays, offsets, hits, True)
         best = VxTuning()
         ms = C.c_float()
-        _check(LIB.vx_batch_autotune(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+        _check(LIB.vx_batch_autotune(self._h, rays.data_ptr(), offs, hits.data_ptr(),
                                      C.byref(best), C.byref(ms), _stream_ptr(stream)), "vx_batch_autotune")
         return best, float(ms.value)
 
     def graph(self, rays, offsets, hits, tuning=None) -> "Graph":
+        """The captured launch reads rays and writes hits at replay: the Graph keeps both tensors (and the
+        batch) referenced until it is closed."""
+        offs = self._args(rays, offsets, hits, True)
         t = _tuning(tuning)
         h = C.c_void_p()
-        _check(LIB.vx_batch_graph_create(self._h, rays.data_ptr(), self._offsets(offsets), hits.data_ptr(),
+        _check(LIB.vx_batch_graph_create(self._h, rays.data_ptr(), offs, hits.data_ptr(),
                                          C.byref(t) if t is not None else None, C.byref(h)), "vx_batch_graph_create")
-        return Graph(h)
+        return Graph(h, keep=(self, rays, hits))
 
 
 class Graph:
     """A captured batched launch (hipGraph) replayed per frame."""
 
-    def __init__(self, h: C.c_void_p):
+    def __init__(self, h: C.c_void_p, keep=()):
         self._h = h
+        self._keep = keep  # buffers the captured kernels use at replay
 
     def launch(self, stream=None) -> None:
         _check(LIB.vx_graph_launch(self._h, _stream_ptr(stream)), "vx_graph_launch")
@@ -748,6 +825,44 @@ class Graph:
         if self._h:
             LIB.vx_graph_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CapturedFrame:
+    """Launches on scenes (vx_trace, vx_trace_lod, vx_render, vx_camera_rays) recorded once into a
+    torch.cuda.CUDAGraph and replayed per frame. The graph holds the scenes' raw device pointers, so this
+    object takes its own reference to every scene (vx_scene_retain) and keeps the borrowed device buffers of
+    device-buffer scenes and the tensors in `keep` alive until close(): a scene may be closed by its creator
+    while the frame is still replayed. record() issues the launches and returns the outputs (kept as
+    .outputs); warm the launches up once before capturing, as torch.cuda.graph expects."""
+
+    def __init__(self, scenes, record, keep=()):
+        torch = _torch()
+        self._scenes = []
+        for sc in scenes:
+            _check(LIB.vx_scene_retain(sc._h), "vx_scene_retain")
+            self._scenes.append(C.c_void_p(sc._h.value))
+        self._keep = (tuple(getattr(sc, "_keep", ()) for sc in scenes), keep)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.outputs = record()
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+    def close(self) -> None:
+        if self._scenes:
+            self.graph = None
+            self.outputs = None
+            for h in self._scenes:
+                LIB.vx_scene_destroy(h)
+            self._scenes = []
+            self._keep = ()
 
     def __del__(self):
         try:
@@ -813,16 +928,18 @@ def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, 
         store_hits = stores_hits(w, h, free_device_bytes(albedo.device), memory_fraction)
     camt = torch.from_numpy(cam.as_floats())
     lt = torch.tensor(np.asarray(light, dtype=np.float32))
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return torch_ext().albedo_render(albedo, int(scene._h.value), camt, int(w), int(h), lt, bool(store_hits),
-                                     bool(deterministic), int(s.cuda_stream))
+    # the binding runs on the current stream (its outputs come from torch's allocator there)
+    with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
+        s = torch.cuda.current_stream()
+        return torch_ext().albedo_render(albedo, int(scene._h.value), camt, int(w), int(h), lt, bool(store_hits),
+                                         bool(deterministic), int(s.cuda_stream))
 
 
 def camera_rays(cam: VxCamera, w: int, h: int, tmax: float, device: int = 0, stream=None, out=None):
     """The w x h pinhole rays of a camera (vx_ray, 32 bytes each), into `out` when given."""
     torch = _torch()
-    if out is not None and out.numel() * out.element_size() < w * h * 32:
-        raise ValueError(f"out holds {out.numel() * out.element_size()} bytes, {w * h * 32} needed")
+    if out is not None:
+        _dev_buf(out, "out", device, w * h * 32)
     rays = out if out is not None else torch.empty(w * h * 32, dtype=torch.uint8, device=torch.device("cuda", device))
     _check(LIB.vx_camera_rays(C.byref(cam), w, h, float(tmax), rays.data_ptr(), _stream_ptr(stream)),
            "vx_camera_rays")

@@ -96,8 +96,9 @@ def build_torch_ext(verbose: bool = False) -> Path | None:
            "-DTORCH_EXTENSION_NAME=vx_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", "-w",
            f"-I{tdir / 'include'}", f"-I{tdir / 'include' / 'torch' / 'csrc' / 'api' / 'include'}",
            f"-I{sysconfig.get_paths()['include']}", f"-I{pybind11.get_include()}", f"-I{INCLUDE}",
+           "-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include",
            str(CSRC / "vx_torch.cpp"), "-o", str(TORCH_EXT.with_suffix(".so.tmp")),
-           f"-L{tdir / 'lib'}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+           f"-L{tdir / 'lib'}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python",
            f"-L{PKG}", "-l:libvixen_hip.so", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{tdir / 'lib'}"]
     if verbose:
         print(" ".join(cmd))

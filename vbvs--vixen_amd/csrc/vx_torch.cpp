@@ -10,7 +10,15 @@
 // the backward pass; without it the backward re-traces the hits.
 // deterministic selects the sorted segmented reduction over FP32 atomics.
 // The discrete hit selection is not differentiated: only albedo gets a gradient.
+//
+// Streams. The tensors made here (image, the stored hits, the gradient) come
+// from torch's caching allocator on the current stream, so the kernels must
+// run on that stream: forward requires `stream` to be the current stream, and
+// backward runs on the current stream of the backward pass (the engine makes
+// it the stream the forward ran on), which is also where torch expects the
+// gradient to be produced.
 #include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
 
 #include <stdexcept>
 #include <string>
@@ -63,6 +71,10 @@ struct AlbedoRender : public torch::autograd::Function<AlbedoRender> {
                     "albedo must be a [V,3] float32 device tensor");
         TORCH_CHECK(light.device().is_cpu() && light.numel() == 3 && light.scalar_type() == torch::kFloat32,
                     "light must be 3 float32 values on the CPU");
+        const hipStream_t cur = c10::hip::getCurrentHIPStream(albedo.device().index()).stream();
+        TORCH_CHECK(reinterpret_cast<hipStream_t>(stream) == cur,
+                    "albedo_render: stream must be the current stream of the albedo's device (enter "
+                    "torch.cuda.stream(s) around the call to use another)");
         auto* sc = reinterpret_cast<vx_scene*>(scene);
         uint64_t voxels = 0;
         check(vx_scene_voxel_count(sc, &voxels), "vx_scene_voxel_count");
@@ -89,7 +101,6 @@ struct AlbedoRender : public torch::autograd::Function<AlbedoRender> {
         ctx->saved_data["height"] = height;
         ctx->saved_data["voxels"] = static_cast<int64_t>(voxels);
         ctx->saved_data["deterministic"] = deterministic;
-        ctx->saved_data["stream"] = stream;
         return image;
     }
 
@@ -104,8 +115,8 @@ struct AlbedoRender : public torch::autograd::Function<AlbedoRender> {
         const int64_t w = ctx->saved_data["width"].toInt(), h = ctx->saved_data["height"].toInt();
         const int64_t voxels = ctx->saved_data["voxels"].toInt();
         const bool det = ctx->saved_data["deterministic"].toBool();
-        const int64_t stream = ctx->saved_data["stream"].toInt();
         auto g = grads[0].contiguous();
+        const hipStream_t stream = c10::hip::getCurrentHIPStream(g.device().index()).stream();
         auto grad = torch::zeros({voxels, 3}, g.options());
         const bool stored = rows.defined();
         check(vx_albedo_backward(sc, &cam, static_cast<uint32_t>(w), static_cast<uint32_t>(h), lt.data_ptr<float>(),
