@@ -52,6 +52,8 @@ def _load() -> C.CDLL:
     lib.orc_shade.argtypes = [C.POINTER(OrcScene), vp, vp, C.c_float]
     lib.orc_shade.restype = C.c_uint32
     lib.orc_render.argtypes = [C.POINTER(OrcScene), vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_int]
+    lib.orc_trace_cpu_reference.argtypes = [C.POINTER(OrcScene), vp, u64, C.c_uint32, vp, vp]
+    lib.orc_trace_cpu_reference.restype = C.c_int
     for f in ("orc_trace", "orc_trace_mt", "orc_trace_lod", "orc_camera_rays", "orc_bc1_decode", "orc_rgba8_decode",
               "orc_bruteforce", "orc_bc1_decode_shader", "orc_dxt_normal_decode", "orc_render"):
         getattr(lib, f).restype = None
@@ -116,6 +118,22 @@ def trace_lod(scene: OracleScene, rays: np.ndarray, ray_orig_size: float, ray_di
     lib().orc_trace_lod(C.byref(scene.s), rays.ctypes.data, len(rays), ray_orig_size, ray_dir_size,
                         out.ctypes.data, max(1, nthreads))
     return out
+
+
+CPU_NO_UNIT_TMAX, CPU_OCTANT_EPS, CPU_CLIP_BRICK, CPU_LOOKUP_INSIDE = 1, 2, 4, 8
+CPU_ALL_ADJUST = 15
+
+
+def trace_cpu_reference(scene: OracleScene, rays: np.ndarray, adjust: int = 0):
+    """The reference's CPU castRay restated for single-level worlds (vx_cpu_reference.c), with the
+    CPU_* adjustments in `adjust`: (records, status per ray: 1 hit, 0 miss, -1 outside the scope)."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    out = np.zeros(len(rays), HIT_DTYPE)
+    status = np.zeros(len(rays), np.int32)
+    if lib().orc_trace_cpu_reference(C.byref(scene.s), rays.ctypes.data, len(rays), int(adjust), out.ctypes.data,
+                                     status.ctypes.data) != 0:
+        raise ValueError("trace_cpu_reference: only worlds of at most 2 bricks per axis in voxel units")
+    return out, status
 
 
 def camera_rays(cam_floats: np.ndarray, w: int, h: int, tmax: float) -> np.ndarray:

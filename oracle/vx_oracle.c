@@ -2,7 +2,7 @@
  * vx_oracle.c -- CPU oracle of the voxel ray-cast hot path. TEST INFRASTRUCTURE:
  * the checker for the HIP kernels, never linked into the product.
  *
- * Built with strict IEEE FP32 (-O2 -ffp-contract=off, no fast-math). Every
+ * Built with strict IEEE FP32 (-O3 -ffp-contract=off, no fast-math). Every
  * expression keeps the evaluation order written here; the HIP kernel must match
  * it bit for bit. min/max follow glm/GLSL: min(a,b) = b < a ? b : a,
  * max(a,b) = a < b ? b : a (libraries/SVO/src/SVOTraversal.cpp uses glm::min/max
@@ -29,8 +29,16 @@
  *                     grid-diagonal distance past it), monotonicity in the cone
  *                     width, and equality with orc_trace where LOD is disabled
  *                     or never fires (tests/test_lod.py)
- *   orc_bruteforce    textbook Amanatides-Woo in FP64; pinned by random-ray
- *                     agreement with an exhaustive slab test per voxel on tiny grids
+ *   orc_bruteforce    textbook Amanatides-Woo in FP64; pinned by agreement with
+ *                     an FP64 slab test against every solid voxel (no stepping)
+ *                     on random rays over tiny dense grids
+ *                     (tests/test_oracle.py::test_bruteforce_pinned_by_slab_test_per_voxel)
+ *   orc_trace_cpu_reference (vx_cpu_reference.c)  the reference's CPU castRay
+ *                     restated verbatim for single-level worlds; equal to
+ *                     trace_one on the reference's own scenario rays, and on
+ *                     random rays equal to it once four named reference steps
+ *                     are replaced, every replaced step siding with the FP64
+ *                     brute force (tests/test_cpu_reference.py)
  */
 #include "vx_oracle.h"
 

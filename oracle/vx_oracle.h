@@ -112,6 +112,19 @@ int orc_setup_ray(const orc_scene* s, const orc_ray* r, float ol[3], float dl[3]
 void orc_bruteforce(const orc_scene* s, const uint32_t* grid, const orc_ray* rays, uint64_t n,
                     double eps, int32_t* vox, double* t, int32_t* ambiguous);
 
+/* The reference's CPU castRay, restated verbatim for single-level worlds (at
+ * most 2 bricks per axis, in voxel units): vx_cpu_reference.c. adjust = 0 is
+ * the reference as written; each ORC_CPU_* bit replaces one step by the
+ * oracle's choice. status[k] is 1 hit, 0 miss, -1 the ray needs a part of the
+ * reference outside that scope. Returns -1 (nothing traced) for a scene
+ * outside the scope. */
+#define ORC_CPU_NO_UNIT_TMAX 1u
+#define ORC_CPU_OCTANT_EPS 2u
+#define ORC_CPU_CLIP_BRICK 4u
+#define ORC_CPU_LOOKUP_INSIDE 8u
+int orc_trace_cpu_reference(const orc_scene* s, const orc_ray* rays, uint64_t n, uint32_t adjust, orc_hit* out,
+                            int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -321,7 +321,8 @@ def _sparse_bruteforce(orc, scene, solid_xyz, rays, eps=2e-5):
                 hi = np.where(inside, hi, -np.inf)
         entry = np.maximum(lo, ts)
         ok = (entry <= hi) & (hi > ts) & (entry <= te)
-        soft = (entry <= hi + eps * np.maximum(1.0, np.abs(hi))) & (hi >= ts - e(ts)) & (entry <= te + e(te))
+        with np.errstate(invalid="ignore"):  # hi = -inf (a parallel axis outside the voxel): not soft
+            soft = (entry <= hi + eps * np.maximum(1.0, np.abs(hi))) & (hi >= ts - e(ts)) & (entry <= te + e(te))
         if not ok.any():
             amb[i] = soft.any()
             continue
@@ -360,3 +361,33 @@ def test_oracle_vs_sparse_bruteforce_maximum_size(vx, orc, N):
     assert hit.mean() > 0.5  # the aimed rays mostly hit
     rel = np.abs(hits["t"][hit].astype(np.float64) - t[hit]) / np.maximum(1.0, np.abs(t[hit]))
     assert rel.max() <= 1e-5
+
+
+# ----------------------------------------------------------------------------
+# The FP64 brute force itself against an exhaustive slab test per voxel
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("N,occ", [(16, 0.05), (16, 0.3), (32, 0.02)])
+def test_bruteforce_pinned_by_slab_test_per_voxel(vx, orc, N, occ):
+    # orc_bruteforce (Amanatides-Woo over the dense grid) and a slab test against every solid voxel (no
+    # stepping at all) must name the same first voxel and entry t on every ray neither calls ambiguous
+    rng = np.random.default_rng(N * 100 + int(occ * 100))
+    grid = np.where(rng.random((N, N, N)) < occ, np.uint32(0xFF336699), np.uint32(0)).astype(np.uint32)
+    built = vx.build_from_dense(grid, bc1=False)
+    scene = orc.OracleScene.from_built(built)
+    z, y, x = np.nonzero(grid)
+    solid = np.stack([x, y, z], 1)
+    n = 1500
+    rays = np.zeros(n, orc.RAY_DTYPE)
+    rays["o"] = rng.uniform(-0.3 * N, 1.3 * N, (n, 3))
+    rays["d"] = rng.normal(size=(n, 3))
+    rays["d"][:100] = np.eye(3)[rng.integers(0, 3, 100)] * rng.choice([-1, 1], (100, 1))  # axis-parallel
+    rays["tmin"] = np.where(rng.random(n) < 0.2, rng.uniform(0, N / 2, n), 0.0)
+    rays["tmax"] = np.where(rng.random(n) < 0.2, rng.uniform(0, N, n), 1e30)
+    vox_aw, t_aw, amb_aw = orc.bruteforce(scene, grid, rays)
+    vox_sl, t_sl, amb_sl = _sparse_bruteforce(orc, scene, solid, rays)
+    clear = ~(amb_aw | amb_sl)
+    assert clear.mean() > 0.95
+    assert (vox_aw[clear] == vox_sl[clear]).all()
+    hit = clear & (vox_sl[:, 0] >= 0)
+    assert hit.sum() > 200
+    assert np.allclose(t_aw[hit], t_sl[hit], rtol=1e-12, atol=1e-12)
