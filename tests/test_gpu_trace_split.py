@@ -1,12 +1,15 @@
-"""vx_trace's two launches: blocks of 64 rays that share one origin are traced in
-lockstep (k_trace), every other block with lane refill over one equal ray range
-per wave (k_trace_refill). Both must give the oracle's records whatever the mix.
+"""vx_trace's launches: a call whose rays give every resident refill wave (about
+7000) at least 128 rays runs as two launches -- blocks of 64 rays that share
+one origin traced in lockstep (k_trace), every other block with lane refill
+over one equal range per wave of a scattered block order (k_trace_refill); a
+shorter call is one k_trace launch that postpones leaf visits in incoherent
+blocks. Every form must give the oracle's records whatever the mix.
 
 The ray sets interleave camera blocks (one origin) with incoherent runs at
 offsets that are not multiples of 64, so some blocks hold both kinds (they are
 incoherent blocks), ranges cut through blocks, and counts are not multiples of
-64. Sizes reach past one range per resident wave (about 7000 waves), so waves
-take several blocks and several refills.
+64. The large sizes reach past 128 rays per resident wave, so they take the
+two-launch form, and waves take several blocks and several refills.
 """
 from __future__ import annotations
 
@@ -40,11 +43,12 @@ def _mixed_rays(orc, cam, w, h, N, rng, n_runs):
     return np.concatenate(parts)
 
 
-@pytest.mark.parametrize("n_runs", [0, 7, 400])
-def test_mixed_coherent_and_incoherent_blocks(vx, orc, cave, gpu_ctx, n_runs):
+@pytest.mark.parametrize("w,h,n_runs", [(333, 219, 0), (333, 219, 7), (333, 219, 400), (1440, 810, 0),
+                                         (1440, 810, 9), (1440, 810, 3000)])
+def test_mixed_coherent_and_incoherent_blocks(vx, orc, cave, gpu_ctx, w, h, n_runs):
     grid, cam, built = cave
     rng = np.random.default_rng(100 + n_runs)
-    rays = _mixed_rays(orc, cam, 333, 219, 256, rng, n_runs)
+    rays = _mixed_rays(orc, cam, w, h, 256, rng, n_runs)
     scene = vx.Scene(gpu_ctx, built)
     got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
     _sync()
@@ -72,12 +76,13 @@ def test_incoherent_counts_and_ranges(vx, orc, cornell_bc1, gpu_ctx, n):
     assert len(bad) == 0, f"n={n}: {len(bad)} records differ, first at {bad[0]}"
 
 
-def test_mixed_blocks_lod(vx, orc, cave, gpu_ctx):
+@pytest.mark.parametrize("w,h,n_runs", [(320, 180, 50), (1440, 810, 2000)])
+def test_mixed_blocks_lod(vx, orc, cave, gpu_ctx, w, h, n_runs):
     import math
     grid, cam, built = cave
     rng = np.random.default_rng(7)
-    rays = _mixed_rays(orc, cam, 320, 180, 256, rng, 50)
-    lod = vx.lod_with_bias(vx.lod_from_camera(math.radians(cam.fov_deg), 180), 4.0)
+    rays = _mixed_rays(orc, cam, w, h, 256, rng, n_runs)
+    lod = vx.lod_with_bias(vx.lod_from_camera(math.radians(cam.fov_deg), h), 4.0)
     scene = vx.Scene(gpu_ctx, built)
     got = vx.hits_to_numpy(scene.trace_lod(vx.rays_to_device(rays), lod))
     _sync()

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <new>
 #include <string>
 #include <vector>
@@ -99,10 +100,25 @@ __global__ void __launch_bounds__(kBlock) k_build_box(const uint32_t* __restrict
     box[i] = lx | ly << 3 | lz << 6 | hx << 9 | hy << 12 | hz << 15;
 }
 
-__device__ __forceinline__ uint32_t xcd_chunk(uint32_t per_xcd)
+// Chunk of workgroup blockIdx.x when the XCDs take stripes of S consecutive
+// chunks in turn: XCD x runs workgroups i = 8k + x, and its k-th takes chunk
+// k % S of stripe 8 (k / S) + x; chunks past the last whole round of stripes go
+// to the XCDs in bands. Neighbouring rays still share an XCD's L2, while every
+// part of the image or ray set -- sky rows and ground rows, a call's camera
+// rays and its incoherent rays -- is spread over every XCD (contiguous bands
+// leave XCDs that drew cheap rays idle while the others finish).
+template <uint32_t S>
+__device__ __forceinline__ uint32_t xcd_stripe_chunk(uint32_t per_xcd)
 {
-    return (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+    const uint32_t full = per_xcd / S * S;
+    if (k < full) return ((k / S) * kXcds + x) * S + k % S;
+    return full * kXcds + x * (per_xcd - full) + (k - full);
 }
+
+// Stripe lengths: k_trace's chunks are 256 consecutive rays, k_render's 16x16 pixel tiles.
+constexpr uint32_t kTraceStripe = 4;
+constexpr uint32_t kRenderStripe = 16;
 
 // Every lane of the workgroup copies part of the descriptor prefix into LDS.
 template <bool STAGED>
@@ -140,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
                                                   unsigned long long* __restrict__ counters)
 {
     stage_prefix<STAGED>(s);
-    const uint64_t i = (uint64_t)xcd_chunk(per_xcd) * kBlock + threadIdx.x;
+    const uint64_t i = (uint64_t)xcd_stripe_chunk<kTraceStripe>(per_xcd) * kBlock + threadIdx.x;
     TraceCounts cnt{0u, 0u, 0u, 0u};
     if (i < n) {
         // a wave holds one aligned 64-ray block; its origins decide how it is traced
@@ -174,29 +190,31 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
 // few lanes stay active. Here a resident grid of waves keeps 64 rays in
 // flight per wave: whenever at least refill_min lanes are idle (between ESVO
 // iterations) they take the next rays of the wave's range (ranked by mbcnt).
-// The rays are split into one contiguous range per wave, of equal length (no
-// atomics or device state, so the launch stays capturable and re-entrant);
-// waves are numbered so each XCD takes a contiguous band of ranges (workgroup
-// i runs on XCD i % 8). A wave walks its range block by block (aligned
-// 64-ray blocks, classified exactly as k_trace classifies them) and skips the
-// blocks that share an origin, which k_trace<..., SPLIT> traces. Leaf visits
-// wait until 3/4 of the live lanes wait at one (trace_ray's POSTPONE). Each
-// lane executes exactly the reference's iteration sequence, so records equal
+// The rays are dealt as one contiguous range of `range` rays per wave, equal
+// in length, in a scattered block order: position p of the dealt order is ray
+// p % 64 of block (p / 64) * step mod n_blocks (step coprime to n_blocks, so a
+// bijection). Clusters of incoherent blocks -- a set of camera rays followed
+// by a set of random rays, say -- then spread evenly over the waves, with no
+// atomics or device state (the launch stays capturable and re-entrant), while
+// each wave still takes its rays 64 neighbours at a time. A wave classifies
+// each block it reaches exactly as k_trace classifies it and skips the blocks
+// that share an origin (k_trace<..., SPLIT> traced them). Leaf visits wait
+// until 3/4 of the live lanes wait at one (trace_ray's POSTPONE). Each lane
+// executes exactly the reference's iteration sequence, so records equal
 // k_trace's bit for bit.
 template <bool STAGED, bool LOD>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 7)))
-    k_trace_refill(SceneView s, const vx_ray* __restrict__ rays, uint64_t n, uint64_t range, uint32_t refill_min,
-                   vx_hit* __restrict__ hits)
+    k_trace_refill(SceneView s, const vx_ray* __restrict__ rays, uint64_t n, uint64_t range, uint64_t step,
+                   uint32_t refill_min, vx_hit* __restrict__ hits)
 {
     stage_prefix<STAGED>(s);
-    constexpr uint32_t wpg = kBlock / 64u;
     const uint32_t lane = threadIdx.x & 63u;
-    // gridDim.x is a multiple of kXcds: XCD x owns waves [x W/8, (x+1) W/8)
-    const uint64_t v = ((uint64_t)(blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds) * wpg +
-                       (threadIdx.x >> 6);
-    uint64_t pos = min(v * range, n);            // next ray of the range not yet classified
-    const uint64_t last = min(pos + range, n);   // end of the range
-    uint64_t cur = pos, end = pos;               // rays of the current block not yet handed out
+    const uint64_t v = (uint64_t)blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
+    const uint64_t n_blocks = (n + 63u) >> 6;
+    const uint64_t slots = n_blocks << 6; // dealt positions: every block counts 64, its rays past n are empty
+    uint64_t pos = min(v * range, slots);        // next dealt position not yet classified
+    const uint64_t last = min(pos + range, slots);
+    uint64_t cur = 0, end = 0;                   // rays of the current block not yet handed out
     bool drained = false;
 
     enum : int { IDLE = 0, ESVO = 1, LEAF = 2 };
@@ -209,13 +227,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 
         uint64_t idle = __ballot(mode == IDLE);
         if (!drained && (uint32_t)__popcll(idle) >= refill_min) {
             while (idle != 0ull) {
-                while (cur == end) { // the next block of the range that k_trace leaves to us
+                while (cur == end) { // the next block of the range that k_trace leaves to this launch
                     if (pos >= last) {
                         drained = true;
                         break;
                     }
-                    const uint64_t b = pos >> 6;
-                    const uint64_t stop = min(min((b + 1u) << 6, n), last);
+                    const uint64_t stop = min((pos | 63u) + 1u, last); // the dealt block's part in the range
+                    const uint64_t b = ((pos >> 6) * step) % n_blocks;
                     const uint64_t ri = (b << 6) + lane;
                     bool shared = true;
                     if (ri < n) {
@@ -223,13 +241,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 
                         const float o[3] = {a.x, a.y, a.z};
                         shared = block_shares_origin(o);
                     }
-                    if (__ballot(!shared) == 0ull) { // every lane of the wave agrees
-                        pos = stop;
-                        continue;
-                    }
-                    cur = pos;
-                    end = stop;
+                    const uint64_t lo = (b << 6) + (pos & 63u), hi = min((b << 6) + ((stop - 1u) & 63u) + 1u, n);
                     pos = stop;
+                    if (__ballot(!shared) == 0ull || lo >= hi) continue; // every lane of the wave agrees
+                    cur = lo;
+                    end = hi;
                 }
                 if (drained) break;
                 const uint64_t take = min((uint64_t)__popcll(idle), end - cur);
@@ -414,7 +430,7 @@ __device__ __forceinline__ void render_tile(const SceneView& s, const CamParams&
                                             vx_hit* __restrict__ hits, uint32_t* __restrict__ rgba)
 {
     stage_prefix<STAGED>(s);
-    const uint32_t tile = xcd_chunk(per_xcd);
+    const uint32_t tile = xcd_stripe_chunk<kRenderStripe>(per_xcd);
     if (tile >= ntiles) return;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
@@ -532,8 +548,12 @@ __global__ void __launch_bounds__(kBlock) k_bounce_rays(const vx_ray* __restrict
     const uint32_t r1 = mix32((uint32_t)i * 2u + 1u + mix32(seed ^ (uint32_t)(i >> 32)));
     const uint32_t r2 = mix32(r1 ^ 0x9e3779b9u);
     const float u1 = (float)(r1 >> 8) * 0x1p-24f, u2 = (float)(r2 >> 8) * 0x1p-24f;
+    // cosine-weighted hemisphere (Malley): a uniform point on the unit disk
+    // (radius sqrt(u1), azimuth 2 pi u2) lifted onto the hemisphere about the normal
     const float r = __builtin_sqrtf(u1), phi = 6.28318530717958648f * u2;
-    const float a1 = r * __cosf(phi), a2 = r * __sinf(phi), up = __builtin_sqrtf(1.0f - u1);
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const float a1 = r * cp, a2 = r * sp, up = __builtin_sqrtf(1.0f - u1);
     const uint32_t t1 = (axis + 1u) % 3u, t2 = (axis + 2u) % 3u;
     float dir[3];
     dir[axis] = sgn * up;
@@ -778,14 +798,20 @@ static SceneView launch_view(const vx_scene* s, uint32_t& lds, bool parks)
 // cave: 16 beats 8, 24 and 32 on random and bounce rays).
 constexpr uint32_t kRefillMin = 16;
 
-// vx_trace as two launches on one stream: k_trace<..., SPLIT> traces the 64-ray
-// blocks that share an origin, k_trace_refill the others. The refill grid is
-// as many workgroups as are resident at once (a multiple of the XCD count),
-// each wave taking an equal contiguous range of at least 64 rays.
+// k_trace_refill pays off once each of its waves is dealt this many rays: with
+// fewer (a call of under about a million rays) a wave has little to refill
+// from, and the two launches' tails cost more than the refill saves, so the
+// call stays one k_trace launch that postpones leaf visits in incoherent blocks.
+constexpr uint64_t kSplitMinRange = 128;
+
+// vx_trace on one stream: one k_trace launch for short calls, else two launches,
+// k_trace<..., SPLIT> for the 64-ray blocks that share an origin and
+// k_trace_refill for the others. Records are the same bits either way.
 template <bool LOD>
-static vx_status launch_trace_split(const vx_scene* s, const vx_lod* lod, const vx_ray* rays, uint64_t n,
-                                    vx_hit* hits, const dim3& grid, uint32_t per, hipStream_t st)
+static vx_status launch_trace_split(const vx_scene* s, const vx_lod* lod, const vx_ray* rays, uint64_t n, vx_hit* hits,
+                                    const dim3& grid, uint32_t per, hipStream_t st)
 {
+    if ((n + 63u) >> 6 >= 0x100000000ull) return fail(VX_ERR_INVALID_ARG, "vx_trace: too many rays for one launch");
     // k_trace<..., SPLIT> never parks a leaf: its lane columns are two words shorter
     uint32_t ldsk = 0, lds = 0;
     SceneView vk = launch_view(s, ldsk, false), v = launch_view(s, lds, true);
@@ -795,21 +821,31 @@ static vx_status launch_trace_split(const vx_scene* s, const vx_lod* lod, const 
     }
     const bool staged = v.lds_count > 0;
     const dim3 block(kBlock);
-    if (vk.lds_count > 0) k_trace<false, true, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
-    else k_trace<false, false, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
-    VX_HIP(hipGetLastError());
     int per_cu = 0;
     if (staged) VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<true, LOD>, kBlock, lds));
     else VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<false, LOD>, kBlock, lds));
     if (per_cu < 1) return fail(VX_ERR_UNSUPPORTED, "vx_trace: the refill kernel does not fit one workgroup per CU");
+    // as many workgroups as are resident at once, each wave dealt an equal range of at least 64 rays, and
+    // fewer workgroups when the rays need fewer waves
+    const uint64_t n_blocks = (n + 63u) >> 6;
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)s->cu_count * (kBlock / 64);
-    const uint64_t range = std::max<uint64_t>(64, (n + resident - 1) / resident);
-    const uint64_t waves = (n + range - 1) / range;
-    uint64_t groups = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    groups = (groups + kXcds - 1) / kXcds * kXcds;
-    const dim3 rgrid(static_cast<unsigned>(groups));
-    if (staged) k_trace_refill<true, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, kRefillMin, hits);
-    else k_trace_refill<false, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, kRefillMin, hits);
+    const uint64_t range = std::max<uint64_t>(64, (n_blocks * 64 + resident - 1) / resident);
+    if (range < kSplitMinRange) {
+        if (staged) k_trace<false, true, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr);
+        else k_trace<false, false, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr);
+        VX_HIP(hipGetLastError());
+        return VX_OK;
+    }
+    if (vk.lds_count > 0) k_trace<false, true, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
+    else k_trace<false, false, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
+    VX_HIP(hipGetLastError());
+    const uint64_t waves = (n_blocks * 64 + range - 1) / range;
+    const dim3 rgrid(static_cast<unsigned>((waves + kBlock / 64 - 1) / (kBlock / 64)));
+    // the block order's step: near n_blocks / golden ratio, odd and coprime to n_blocks
+    uint64_t step = (uint64_t)((double)n_blocks * 0.6180339887498949) | 1u;
+    while (std::gcd(step, n_blocks) != 1u) step += 2u;
+    if (staged) k_trace_refill<true, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, step % n_blocks, kRefillMin, hits);
+    else k_trace_refill<false, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, step % n_blocks, kRefillMin, hits);
     VX_HIP(hipGetLastError());
     return VX_OK;
 }
