@@ -39,6 +39,9 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp", "vx_cache.cpp"]
+# host code that calls the HIP runtime API (no kernels): the C++ ISVOStructure adapter
+HIP_HOST_SOURCES = ["vx_svo.cpp"]
+HIP_HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include"]
 HIP_SOURCES = ["vx_api.hip", "vx_batch.hip", "vx_albedo.hip"]
 
 
@@ -59,6 +62,13 @@ def build(verbose: bool = False) -> Path:
     for src in HOST_SOURCES:
         out = OBJ / (Path(src).stem + ".o")
         cmd = [CXX, *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(out)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(out))
+    for src in HIP_HOST_SOURCES:
+        out = OBJ / (Path(src).stem + ".o")
+        cmd = [CXX, *HOST_FLAGS, *HIP_HOST_FLAGS, "-c", str(CSRC / src), "-o", str(out)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
