@@ -44,8 +44,22 @@ typedef struct vx_built_scene {
     vx_octree_config config;
 } vx_built_scene;
 
-/* Build from a dense grid grid[x + N*(y + N*z)] of RGBA8 words (0 = empty).
- * N must be a power of two in [16, 4096]. nthreads <= 0 uses all cores. */
+/* What the BC1 pool is encoded from (with_bc1 of the builders):
+ *   VX_BC1_NONE          no BC1 pool;
+ *   VX_BC1_FROM_RGBA8    each voxel word is an RGBA8 colour (R in bits 0-7);
+ *   VX_BC1_FROM_MATERIAL each voxel word is a material id, coloured by
+ *                        MaterialIdToColor (vx_material_id_to_color), as
+ *                        LaineKarrasOctree::rebuild encodes VIXEN's scenes
+ *                        (libraries/SVO/src/SVORebuild.cpp:680-715).
+ * brickData holds the words as given either way (VIXEN stores material ids,
+ * VoxelSceneCacher.cpp:551-588). */
+#define VX_BC1_NONE 0
+#define VX_BC1_FROM_RGBA8 1
+#define VX_BC1_FROM_MATERIAL 2
+
+/* Build from a dense grid grid[x + N*(y + N*z)] of voxel words (0 = empty):
+ * RGBA8 colours or material ids (see VX_BC1_*). N must be a power of two in
+ * [16, 4096]. nthreads <= 0 uses all cores. */
 vx_status vx_build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1, int nthreads,
                               vx_built_scene** out);
 
@@ -65,6 +79,11 @@ void vx_built_scene_free(vx_built_scene* s);
 /* BC1 encode of 16 colours in [0,1]^3 (DXT1Compressor::encodeBlockTyped with
  * validCount 16). */
 uint64_t vx_bc1_encode_block(const float colors[16][3]);
+
+/* MaterialIdToColor (libraries/SVO/src/SVORebuild.cpp:68-120), the CPU table
+ * the rebuild encodes BC1 colours from: getMaterialColor's constants except for
+ * ids 33-39, which it lists instead of interpolating, and the same HSV wheel. */
+void vx_material_id_to_color(uint32_t material_id, float rgb[3]);
 
 /* The normal pool of VIXEN's compressed shader (compressedNormals, binding 7):
  * per brick 32 blocks of 16 consecutive linear voxels, each block two u64

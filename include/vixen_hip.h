@@ -154,7 +154,24 @@ typedef struct vx_scene_desc {
     const vx_octree_config* config;
     const uint64_t* dxt_normals;  /* brick_count x 32 DXT normal blocks of two u64
                                      (compressedNormals, see vx_encode_normals), or NULL */
+    const float* material_palette; /* 256 x RGB, or NULL (a host pointer, copied at
+                                     creation, also for vx_scene_create_device). With no
+                                     BC1 colours it makes a hit's colour
+                                     palette[brickData & 0xFF], the uncompressed
+                                     pipeline's getMaterialColor(brickData & 0xFF)
+                                     (shaders/VoxelRayMarch.comp:189-191) when it holds
+                                     vx_material_palette's table; BC1 colours, when
+                                     present, take precedence as in the compressed
+                                     pipeline */
 } vx_scene_desc;
+
+/* getMaterialColor (shaders/Materials.glsl:18-62) of a material id, in FP32:
+ * the table's constants, mix(a, b, t) = a (1 - t) + b t for ids 33-39, and the
+ * HSV wheel (s = 0.7, v = 0.8) for every other id. Host-only arithmetic. */
+void vx_material_color(uint32_t material_id, float rgb[3]);
+/* vx_material_color of ids 0..255, RGB per id: the palette that makes
+ * vx_scene_desc.material_palette give VIXEN's uncompressed colours. */
+void vx_material_palette(float palette[256 * 3]);
 
 typedef struct vx_ctx vx_ctx;
 typedef struct vx_scene vx_scene;

@@ -25,7 +25,7 @@ class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("brick_data", C.c_void_p),
                 ("brick_count", C.c_uint64), ("bc1", C.c_void_p), ("grid_min", C.c_float * 3),
                 ("grid_max", C.c_float * 3), ("max_levels", C.c_int32), ("bricks_per_axis", C.c_int32),
-                ("normals", C.c_void_p)]
+                ("normals", C.c_void_p), ("palette", C.c_void_p)]
 
 
 def build() -> Path:
@@ -54,8 +54,11 @@ def _load() -> C.CDLL:
     lib.orc_render.argtypes = [C.POINTER(OrcScene), vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_int]
     lib.orc_trace_cpu_reference.argtypes = [C.POINTER(OrcScene), vp, u64, C.c_uint32, vp, vp]
     lib.orc_trace_cpu_reference.restype = C.c_int
+    lib.orc_material_color.argtypes = [C.c_uint32, vp]
+    lib.orc_material_id_to_color.argtypes = [C.c_uint32, vp]
     for f in ("orc_trace", "orc_trace_mt", "orc_trace_lod", "orc_camera_rays", "orc_bc1_decode", "orc_rgba8_decode",
-              "orc_bruteforce", "orc_bc1_decode_shader", "orc_dxt_normal_decode", "orc_render"):
+              "orc_bruteforce", "orc_bc1_decode_shader", "orc_dxt_normal_decode", "orc_render", "orc_material_color",
+              "orc_material_id_to_color"):
         getattr(lib, f).restype = None
     return lib
 
@@ -73,7 +76,8 @@ def lib() -> C.CDLL:
 class OracleScene:
     """Holds the arrays an orc_scene points at (nodes, bricks, BC1, bounds)."""
 
-    def __init__(self, nodes, brick_data, bc1, grid_min, grid_max, max_levels, bricks_per_axis, normals=None):
+    def __init__(self, nodes, brick_data, bc1, grid_min, grid_max, max_levels, bricks_per_axis, normals=None,
+                 palette=None):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         self.brick_data = np.ascontiguousarray(brick_data, dtype=np.uint32)
         self.bc1 = None if bc1 is None else np.ascontiguousarray(bc1, dtype=np.uint64)
@@ -89,13 +93,16 @@ class OracleScene:
         s.max_levels = int(max_levels)
         s.bricks_per_axis = int(bricks_per_axis)
         s.normals = self.normals.ctypes.data if (self.normals is not None and self.normals.size) else None
+        self.palette = None if palette is None else np.ascontiguousarray(palette, dtype=np.float32).reshape(256, 3)
+        s.palette = self.palette.ctypes.data if self.palette is not None else None
         self.s = s
 
     @classmethod
     def from_built(cls, built) -> "OracleScene":
         cfg = built.config
         return cls(built.nodes, built.brick_data, built.bc1, list(cfg.grid_min), list(cfg.grid_max),
-                   cfg.user_max_levels, cfg.bricks_per_axis, getattr(built, "normals", None))
+                   cfg.user_max_levels, cfg.bricks_per_axis, getattr(built, "normals", None),
+                   getattr(built, "palette", None))
 
 
 def trace(scene: OracleScene, rays: np.ndarray, nthreads: int = 1):
@@ -174,6 +181,25 @@ def rgba8_decode(word: int) -> np.ndarray:
     rgb = np.zeros(3, np.float32)
     lib().orc_rgba8_decode(int(word), rgb.ctypes.data)
     return rgb
+
+
+def material_color(material_id: int) -> np.ndarray:
+    """getMaterialColor (shaders/Materials.glsl:18-62)."""
+    rgb = np.zeros(3, np.float32)
+    lib().orc_material_color(int(material_id), rgb.ctypes.data)
+    return rgb
+
+
+def material_id_to_color(material_id: int) -> np.ndarray:
+    """MaterialIdToColor (libraries/SVO/src/SVORebuild.cpp:68-120)."""
+    rgb = np.zeros(3, np.float32)
+    lib().orc_material_id_to_color(int(material_id), rgb.ctypes.data)
+    return rgb
+
+
+def material_palette() -> np.ndarray:
+    """material_color of ids 0..255, (256, 3) float32."""
+    return np.stack([material_color(i) for i in range(256)])
 
 
 def setup_ray(scene: OracleScene, ray) -> tuple | None:

@@ -392,8 +392,7 @@ static int cast_one(const orc_scene* s, const orc_ray* r, orc_hit* h, uint32_t a
                     h->leaf_node = 0xFFFFFFFFu;
                     h->brick = (uint32_t)brick;
                     h->voxel = voxel;
-                    if (s->bc1) orc_bc1_decode(s->bc1[(uint64_t)brick * 32u + (voxel >> 4)], (int)(voxel & 15u), h->rgb);
-                    else orc_rgba8_decode(s->brick_data[(uint64_t)brick * 512u + voxel], h->rgb);
+                    orc_voxel_color(s, brick, voxel, s->brick_data[(uint64_t)brick * 512u + voxel], h->rgb);
                     return 1;
                 }
             }

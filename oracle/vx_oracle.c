@@ -33,6 +33,10 @@
  *                     an FP64 slab test against every solid voxel (no stepping)
  *                     on random rays over tiny dense grids
  *                     (tests/test_oracle.py::test_bruteforce_pinned_by_slab_test_per_voxel)
+ *   orc_material_color, orc_material_id_to_color  pinned by the tables'
+ *                     constants typed from shaders/Materials.glsl and
+ *                     SVORebuild.cpp, the mix of ids 33-39 and the HSV wheel
+ *                     against float64 closed forms (tests/test_materials.py)
  *   orc_trace_cpu_reference (vx_cpu_reference.c)  the reference's CPU castRay
  *                     restated verbatim for single-level worlds; equal to
  *                     trace_one on the reference's own scenario rays, and on
@@ -94,6 +98,98 @@ void orc_rgba8_decode(uint32_t w, float rgb[3])
     rgb[0] = (float)(w & 255u) / 255.0f;
     rgb[1] = (float)((w >> 8) & 255u) / 255.0f;
     rgb[2] = (float)((w >> 16) & 255u) / 255.0f;
+}
+
+void orc_voxel_color(const orc_scene* s, uint32_t brick, uint32_t voxel, uint32_t word, float rgb[3])
+{
+    if (s->bc1) {
+        orc_bc1_decode(s->bc1[(uint64_t)brick * 32u + (voxel >> 4)], (int)(voxel & 15u), rgb);
+    } else if (s->palette) {
+        const float* c = s->palette + 3u * (word & 0xFFu);
+        rgb[0] = c[0];
+        rgb[1] = c[1];
+        rgb[2] = c[2];
+    } else {
+        orc_rgba8_decode(word, rgb);
+    }
+}
+
+static void rgb3(float rgb[3], float r, float g, float b)
+{
+    rgb[0] = r;
+    rgb[1] = g;
+    rgb[2] = b;
+}
+
+/* The default branch both material tables end with: an HSV wheel, value 0.8,
+ * saturation 0.7, hue (id mod 256) / 256. */
+static void material_hsv(uint32_t id, float rgb[3])
+{
+    float hue = (float)(id % 256u) / 256.0f;
+    float h = hue * 6.0f;
+    int i = (int)h;
+    float f = h - (float)i;
+    float p = 0.8f * 0.3f;
+    float q = 0.8f * (1.0f - 0.7f * f);
+    float t = 0.8f * (1.0f - 0.7f * (1.0f - f));
+    if (i == 0) rgb3(rgb, 0.8f, t, p);
+    else if (i == 1) rgb3(rgb, q, 0.8f, p);
+    else if (i == 2) rgb3(rgb, p, 0.8f, t);
+    else if (i == 3) rgb3(rgb, p, q, 0.8f);
+    else if (i == 4) rgb3(rgb, t, p, 0.8f);
+    else rgb3(rgb, 0.8f, p, q);
+}
+
+/* The constants both tables share; 0 when the id has none. */
+static int material_const(uint32_t id, float rgb[3])
+{
+    switch (id) {
+    case 1: rgb3(rgb, 1.0f, 0.0f, 0.0f); return 1;
+    case 2: rgb3(rgb, 0.0f, 1.0f, 0.0f); return 1;
+    case 3: rgb3(rgb, 0.9f, 0.9f, 0.9f); return 1;
+    case 4: rgb3(rgb, 0.85f, 0.85f, 0.85f); return 1;
+    case 5: rgb3(rgb, 0.95f, 0.95f, 0.95f); return 1;
+    case 6: rgb3(rgb, 0.8f, 0.8f, 0.8f); return 1;
+    case 7: rgb3(rgb, 0.4f, 0.4f, 0.4f); return 1;
+    case 10: rgb3(rgb, 0.8f, 0.6f, 0.2f); return 1;
+    case 11: rgb3(rgb, 0.6f, 0.8f, 0.9f); return 1;
+    case 20: rgb3(rgb, 1.0f, 0.98f, 0.9f); return 1;
+    case 30: rgb3(rgb, 0.5f, 0.45f, 0.4f); return 1;
+    case 31: rgb3(rgb, 0.6f, 0.55f, 0.5f); return 1;
+    case 32: rgb3(rgb, 0.55f, 0.5f, 0.45f); return 1;
+    case 40: rgb3(rgb, 0.8f, 0.6f, 0.2f); return 1;
+    case 50: rgb3(rgb, 0.2f, 0.2f, 0.22f); return 1;
+    case 60: rgb3(rgb, 0.6f, 0.58f, 0.55f); return 1;
+    case 61: rgb3(rgb, 0.7f, 0.85f, 0.95f); return 1;
+    default: return 0;
+    }
+}
+
+void orc_material_color(uint32_t id, float rgb[3])
+{
+    if (material_const(id, rgb)) return;
+    if (id >= 33u && id <= 39u) {
+        /* mix(x, y, a) = x * (1 - a) + y * a (GLSL 4.60, 8.3) */
+        float t = (float)(id - 33u) / 6.0f;
+        rgb3(rgb, 0.52f * (1.0f - t) + 0.68f * t, 0.48f * (1.0f - t) + 0.6f * t, 0.42f * (1.0f - t) + 0.52f * t);
+        return;
+    }
+    material_hsv(id, rgb);
+}
+
+void orc_material_id_to_color(uint32_t id, float rgb[3])
+{
+    if (material_const(id, rgb)) return;
+    switch (id) {
+    case 33: rgb3(rgb, 0.52f, 0.48f, 0.42f); return;
+    case 34: rgb3(rgb, 0.48f, 0.44f, 0.38f); return;
+    case 35: rgb3(rgb, 0.45f, 0.42f, 0.36f); return;
+    case 36: rgb3(rgb, 0.58f, 0.52f, 0.46f); return;
+    case 37: rgb3(rgb, 0.62f, 0.56f, 0.48f); return;
+    case 38: rgb3(rgb, 0.65f, 0.58f, 0.5f); return;
+    case 39: rgb3(rgb, 0.68f, 0.6f, 0.52f); return;
+    default: material_hsv(id, rgb); return;
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -247,8 +343,7 @@ static int brick_dda(const orc_scene* s, uint32_t brick, const int bc[3], const 
             h->t = tHit;
             h->brick = brick;
             h->voxel = lin;
-            if (s->bc1) orc_bc1_decode(s->bc1[(uint64_t)brick * 32u + (lin >> 4)], (int)(lin & 15u), h->rgb);
-            else orc_rgba8_decode(w, h->rgb);
+            orc_voxel_color(s, brick, lin, w, h->rgb);
             return 1;
         }
         if (tNext[0] < tNext[1] && tNext[0] < tNext[2]) {

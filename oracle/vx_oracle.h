@@ -55,6 +55,7 @@ typedef struct orc_scene {
     int32_t max_levels;         /* log2(N) */
     int32_t bricks_per_axis;    /* N / 8 */
     const uint64_t* normals;    /* brick_count x 32 DXT normal blocks (2 u64 each), or NULL */
+    const float* palette;       /* 256 x RGB by material id (word & 0xFF), or NULL */
 } orc_scene;
 
 /* counts[0] rays, [1] ESVO iterations, [2] brick visits, [3] DDA voxels, [4] hits */
@@ -96,6 +97,17 @@ void orc_render(const orc_scene* s, const float cam[16], uint32_t w, uint32_t h,
 
 /* RGBA8 colour of an uncompressed voxel word. */
 void orc_rgba8_decode(uint32_t word, float rgb[3]);
+
+/* Colour of the hit voxel (brick, linear index, its word): the BC1 texel when
+ * the scene has BC1 blocks, else palette[word & 0xFF] when it has a palette,
+ * else the word's RGBA8 colour. */
+void orc_voxel_color(const orc_scene* s, uint32_t brick, uint32_t voxel, uint32_t word, float rgb[3]);
+
+/* getMaterialColor (shaders/Materials.glsl:18-62) in FP32. */
+void orc_material_color(uint32_t id, float rgb[3]);
+
+/* MaterialIdToColor (libraries/SVO/src/SVORebuild.cpp:68-120) in FP32. */
+void orc_material_id_to_color(uint32_t id, float rgb[3]);
 
 /* Ray set-up shared by the traversal and the brute force: the local-space
  * ray (voxel units, grid origin at 0), the traced segment [tstart, texit]

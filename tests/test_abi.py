@@ -50,7 +50,8 @@ _Static_assert(offsetof(vx_octree_config, grid_min) == 32 && offsetof(vx_octree_
                offsetof(vx_octree_config, _padding4) == 192, "config offsets");
 _Static_assert(offsetof(vx_hit, rgb) == 20 && offsetof(vx_ray, d) == 16, "record offsets");
 _Static_assert(offsetof(vx_scene_desc, config) == 40 && offsetof(vx_scene_desc, dxt_normals) == 48 &&
-               sizeof(vx_scene_desc) == 56, "scene descriptor");
+               offsetof(vx_scene_desc, material_palette) == 56 && sizeof(vx_scene_desc) == 64, "scene descriptor");
+_Static_assert(VX_BC1_NONE == 0 && VX_BC1_FROM_RGBA8 == 1 && VX_BC1_FROM_MATERIAL == 2, "BC1 sources");
 int main(void) { return 0; }
 """)
     r = subprocess.run(["gcc", "-std=c11", "-fsyntax-only", f"-I{ROOT / 'include'}", str(src)],
@@ -62,7 +63,8 @@ def test_numpy_dtypes_match_abi(vx):
     assert vx.RAY_DTYPE.itemsize == C.sizeof(C.c_float) * 8
     assert vx.HIT_DTYPE.fields["rgb"][1] == 20
     assert C.sizeof(vx.VxOctreeConfig) == 256 and C.sizeof(vx.VxCamera) == 64
-    assert C.sizeof(vx.VxSceneDesc) == 56 and vx.VxSceneDesc.dxt_normals.offset == 48
+    assert C.sizeof(vx.VxSceneDesc) == 64 and vx.VxSceneDesc.dxt_normals.offset == 48
+    assert vx.VxSceneDesc.material_palette.offset == 56
 
 
 def test_version_and_no_device_errors(vx):

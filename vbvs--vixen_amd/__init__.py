@@ -86,7 +86,8 @@ class VxCamera(C.Structure):
 class VxSceneDesc(C.Structure):
     _fields_ = [("esvo_nodes", C.c_void_p), ("node_count", C.c_uint64),
                 ("brick_data", C.c_void_p), ("brick_count", C.c_uint64),
-                ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig)), ("dxt_normals", C.c_void_p)]
+                ("bc1_colors", C.c_void_p), ("config", C.POINTER(VxOctreeConfig)), ("dxt_normals", C.c_void_p),
+                ("material_palette", C.c_void_p)]
 
 
 class VxCacheEntryInfo(C.Structure):
@@ -164,6 +165,9 @@ def _load() -> C.CDLL:
         "vx_lod_from_camera": ([C.c_float, i32], VxLod),
         "vx_lod_from_camera_near": ([C.c_float, i32, C.c_float], VxLod),
         "vx_lod_with_bias": ([VxLod, C.c_float], VxLod),
+        "vx_material_color": ([u32, vp], None),
+        "vx_material_palette": ([vp], None),
+        "vx_material_id_to_color": ([u32, vp], None),
         "vx_render": ([vp, C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp, vp], st),
         "vx_camera_rays": ([C.POINTER(VxCamera), u32, u32, C.c_float, vp, vp], st),
         "vx_bounce_rays": ([vp, vp, vp, u64, u32, vp, vp], st),
@@ -215,7 +219,7 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
             "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
             "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device",
-            "vx_albedo_backward_noalloc")
+            "vx_albedo_backward_noalloc", "vx_material_color", "vx_material_palette", "vx_material_id_to_color")
 
 
 def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:
@@ -252,6 +256,7 @@ class BuiltScene:
     internal_count: int
     config: VxOctreeConfig
     normals: np.ndarray | None = None  # (brick_count, 64) uint64 DXT normal blocks, see encode_normals
+    palette: np.ndarray | None = None  # (256, 3) float32 colour per material id (brick word & 0xFF), or None
 
     @property
     def N(self) -> int:
@@ -312,27 +317,66 @@ def seeded_views(grid: np.ndarray, seed: int, n: int, fov_deg: float = 70.0, asp
     return cams
 
 
-def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0) -> BuiltScene:
-    """Dense RGBA8 grid grid[z, y, x] (0 = empty) -> octree buffers."""
+# what the builders encode BC1 blocks from (VX_BC1_* of vixen_build.h)
+BC1_NONE, BC1_FROM_RGBA8, BC1_FROM_MATERIAL = 0, 1, 2
+
+
+def _bc1_mode(bc1: bool, materials: bool) -> int:
+    return (BC1_FROM_MATERIAL if materials else BC1_FROM_RGBA8) if bc1 else BC1_NONE
+
+
+def build_from_dense(grid: np.ndarray, bc1: bool = True, nthreads: int = 0, materials: bool = False) -> BuiltScene:
+    """Dense grid grid[z, y, x] (0 = empty) -> octree buffers. The words are RGBA8 colours, or with
+    materials=True VIXEN's material ids: BC1 blocks (bc1=True) then come from MaterialIdToColor, as VIXEN's
+    rebuild encodes them; without BC1 the scene carries getMaterialColor's palette (the uncompressed
+    pipeline's colour)."""
     grid = np.ascontiguousarray(grid, dtype=np.uint32)
     N = grid.shape[0]
     assert grid.shape == (N, N, N)
     out = C.POINTER(VxBuiltScene)()
-    _check(LIB.vx_build_from_dense(_ptr(grid), N, int(bc1), nthreads, C.byref(out)), "vx_build_from_dense")
-    return _take_built(out, bc1)
+    _check(LIB.vx_build_from_dense(_ptr(grid), N, _bc1_mode(bc1, materials), nthreads, C.byref(out)),
+           "vx_build_from_dense")
+    built = _take_built(out, bc1)
+    if materials and not bc1:
+        built.palette = material_palette()
+    return built
 
 
 def build_from_bricks(cells: np.ndarray, voxels: np.ndarray, N: int, bc1: bool = True,
-                      nthreads: int = 0) -> BuiltScene:
+                      nthreads: int = 0, materials: bool = False) -> BuiltScene:
     """Sparse world -> octree buffers: cells [n, 3] brick cells (x, y, z) in [0, N/8),
-    voxels [n, 8, 8, 8] RGBA8 as voxels[i, z, y, x] (0 = empty)."""
+    voxels [n, 8, 8, 8] as voxels[i, z, y, x] (0 = empty), RGBA8 or material ids as in build_from_dense."""
     cells = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1, 3)
     voxels = np.ascontiguousarray(voxels, dtype=np.uint32).reshape(-1, 512)
     assert len(cells) == len(voxels)
     out = C.POINTER(VxBuiltScene)()
-    _check(LIB.vx_build_from_bricks(_ptr(cells), _ptr(voxels), len(cells), N, int(bc1), nthreads, C.byref(out)),
-           "vx_build_from_bricks")
-    return _take_built(out, bc1)
+    _check(LIB.vx_build_from_bricks(_ptr(cells), _ptr(voxels), len(cells), N, _bc1_mode(bc1, materials), nthreads,
+                                    C.byref(out)), "vx_build_from_bricks")
+    built = _take_built(out, bc1)
+    if materials and not bc1:
+        built.palette = material_palette()
+    return built
+
+
+def material_palette() -> np.ndarray:
+    """getMaterialColor (shaders/Materials.glsl:18-62) of ids 0..255, as (256, 3) float32."""
+    out = np.zeros((256, 3), np.float32)
+    LIB.vx_material_palette(_ptr(out))
+    return out
+
+
+def material_color(material_id: int) -> np.ndarray:
+    """getMaterialColor (shaders/Materials.glsl:18-62) of one material id."""
+    out = np.zeros(3, np.float32)
+    LIB.vx_material_color(int(material_id), _ptr(out))
+    return out
+
+
+def material_id_to_color(material_id: int) -> np.ndarray:
+    """MaterialIdToColor (libraries/SVO/src/SVORebuild.cpp:68-120) of one material id."""
+    out = np.zeros(3, np.float32)
+    LIB.vx_material_id_to_color(int(material_id), _ptr(out))
+    return out
 
 
 def _take_built(out, bc1: bool) -> BuiltScene:
@@ -377,6 +421,12 @@ def _scene_desc(built: BuiltScene) -> tuple[VxSceneDesc, VxOctreeConfig]:
     desc.brick_count = built.brick_count
     desc.bc1_colors = _ptr(built.bc1) if (built.bc1 is not None and built.brick_count) else None
     desc.dxt_normals = _ptr(built.normals) if (built.normals is not None and built.brick_count) else None
+    if built.palette is not None:
+        pal = np.ascontiguousarray(built.palette, dtype=np.float32)
+        if pal.shape != (256, 3):
+            raise ValueError("palette: a (256, 3) float32 array of RGB per material id")
+        built.palette = pal
+        desc.material_palette = _ptr(pal)
     cfg = built.config
     desc.config = C.pointer(cfg)
     return desc, cfg
@@ -404,7 +454,7 @@ def shader_frame(built: BuiltScene, world_grid_size: float = WORLD_GRID_SIZE) ->
     for i in range(3):
         cfg.grid_min[i], cfg.grid_max[i] = 0.0, float(world_grid_size)
     return BuiltScene(built.nodes, built.brick_data, built.bc1, built.brick_coords, built.internal_count, cfg,
-                      built.normals)
+                      built.normals, built.palette)
 
 
 def write_cache(path, entries) -> None:
@@ -605,12 +655,14 @@ class Scene:
         self._h = h
 
     @classmethod
-    def from_device(cls, ctx: Context, nodes, brick_data, config: VxOctreeConfig, bc1=None, normals=None) -> "Scene":
+    def from_device(cls, ctx: Context, nodes, brick_data, config: VxOctreeConfig, bc1=None, normals=None,
+                    palette=None) -> "Scene":
         """A scene over caller-owned device buffers (vx_scene_create_device), e.g. CUDA tensors holding the
         descriptor pairs, brick words, BC1 blocks and DXT normal blocks. They are used in place: the scene keeps
         them referenced, and they must stay unchanged until close() (the kernels read them without range
         checks; the tree was validated once, at creation). Each must be a contiguous CUDA tensor on the
-        context's device; bc1 and normals must hold 256 and 512 bytes per brick."""
+        context's device; bc1 and normals must hold 256 and 512 bytes per brick. palette (host, (256, 3)
+        float32, e.g. material_palette()) is copied."""
         self = cls.__new__(cls)
         self.ctx, self.built = ctx, None
         dev = ctx.device
@@ -623,8 +675,14 @@ class Scene:
             _dev_buf(normals, "normals", dev, 512 * brick_count)
         self._cfg = VxOctreeConfig()
         C.memmove(C.addressof(self._cfg), C.addressof(config), C.sizeof(config))
+        pal = None
+        if palette is not None:
+            pal = np.ascontiguousarray(palette, dtype=np.float32)
+            if pal.shape != (256, 3):
+                raise ValueError("palette: a (256, 3) float32 array of RGB per material id")
         self._keep = (nodes, brick_data, bc1, normals)
         desc = VxSceneDesc()
+        desc.material_palette = _ptr(pal) if pal is not None else None
         desc.esvo_nodes = nodes.data_ptr()
         desc.node_count = nodes.numel() * nodes.element_size() // 8
         desc.brick_data = brick_data.data_ptr() if brick_data.numel() else None

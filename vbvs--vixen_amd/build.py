@@ -38,7 +38,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp", "vx_cache.cpp"]
+HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp", "vx_cache.cpp", "vx_materials.cpp"]
 # host code that calls the HIP runtime API (no kernels): the C++ ISVOStructure adapter
 HIP_HOST_SOURCES = ["vx_svo.cpp"]
 HIP_HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include"]

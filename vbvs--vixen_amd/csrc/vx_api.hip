@@ -713,6 +713,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
         }
         (void)hipFree(s->occ);
         (void)hipFree(s->box);
+        (void)hipFree(s->palette);
         delete s;
         return st;
     };
@@ -720,8 +721,12 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     const size_t node_bytes = d->node_count * 8, brick_bytes = std::max<uint64_t>(1, nb) * 2048,
                  occ_bytes = std::max<uint64_t>(1, nb) * 64, bc1_bytes = std::max<uint64_t>(1, nb) * 256,
                  nrm_bytes = std::max<uint64_t>(1, nb) * 512;
-    if (hipMalloc(&s->occ, occ_bytes) != hipSuccess || hipMalloc(&s->box, std::max<uint64_t>(1, nb) * 4) != hipSuccess)
+    if (hipMalloc(&s->occ, occ_bytes) != hipSuccess || hipMalloc(&s->box, std::max<uint64_t>(1, nb) * 4) != hipSuccess ||
+        (d->material_palette && hipMalloc(&s->palette, 256 * 3 * sizeof(float)) != hipSuccess))
         return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_scene_create: device allocation failed"));
+    if (d->material_palette &&
+        hipMemcpy(s->palette, d->material_palette, 256 * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(VX_ERR_HIP, "vx_scene_create: palette upload failed"));
     if (device) {
         s->nodes = const_cast<uint2*>(reinterpret_cast<const uint2*>(d->esvo_nodes));
         s->brick_data = const_cast<uint32_t*>(d->brick_data);
@@ -752,6 +757,7 @@ static vx_status scene_create(vx_ctx* ctx, const vx_scene_desc* d, vx_scene** ou
     v.box = s->box;
     v.bc1 = d->bc1_colors ? s->bc1 : nullptr;
     v.nrm = d->dxt_normals ? s->nrm : nullptr;
+    v.palette = s->palette;
     v.node_count = static_cast<uint32_t>(d->node_count);
     v.brick_count = static_cast<uint32_t>(nb);
     const uint32_t N = B * 8;
@@ -965,6 +971,7 @@ vx_status vx_scene_destroy(vx_scene* s)
     }
     (void)hipFree(s->occ);
     (void)hipFree(s->box);
+    (void)hipFree(s->palette);
     delete s;
     return VX_OK;
 }

@@ -6,8 +6,9 @@
 // same init_ray/esvo_step/visit_leaf code as vx_trace, so results are
 // bit-identical to per-scene launches.
 //
-// Memory: the scenes' descriptor, brick, occupancy and BC1 pools are copied
-// into four concatenated device pools; each scene's SceneView points at its
+// Memory: the scenes' descriptor, brick, occupancy, box and BC1 pools and
+// material palettes are copied into concatenated device pools; each scene's
+// SceneView points at its
 // own slice (per-scene offsets), so descriptor and brick indices stay
 // scene-relative and hit records match the per-scene ones byte for byte.
 //
@@ -191,6 +192,7 @@ struct vx_batch {
     uint32_t* occ = nullptr;
     uint32_t* box = nullptr;
     uint2* bc1 = nullptr;
+    float* palette = nullptr; // 256 x RGB per scene that has a material palette
     // launch state of vx_trace_batched
     vx::BatchSceneInfo* d_table = nullptr;
     unsigned int* d_queue = nullptr;
@@ -322,7 +324,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
     if (!ctx || !scenes || !out || n == 0) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: bad arguments");
     *out = nullptr;
     VX_DEVICE_GUARD(ctx->device);
-    uint64_t nodes = 0, bricks = 0, bc1_bricks = 0;
+    uint64_t nodes = 0, bricks = 0, bc1_bricks = 0, palettes = 0;
     for (uint32_t s = 0; s < n; ++s) {
         if (!scenes[s]) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: null scene");
         if (scenes[s]->device != ctx->device)
@@ -332,6 +334,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         nodes += scenes[s]->node_count;
         bricks += scenes[s]->brick_count;
         if (scenes[s]->view.bc1) bc1_bricks += scenes[s]->brick_count;
+        if (scenes[s]->palette) ++palettes;
     }
     auto* b = new (std::nothrow) vx_batch{};
     if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: out of memory");
@@ -344,6 +347,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         (void)hipFree(b->occ);
         (void)hipFree(b->box);
         (void)hipFree(b->bc1);
+        (void)hipFree(b->palette);
         (void)hipFree(b->d_table);
         (void)hipFree(b->d_queue);
         delete b;
@@ -354,10 +358,11 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         hipMalloc(&b->occ, std::max<uint64_t>(1, bricks) * 64) != hipSuccess ||
         hipMalloc(&b->box, std::max<uint64_t>(1, bricks) * 4) != hipSuccess ||
         hipMalloc(&b->bc1, std::max<uint64_t>(1, bc1_bricks) * 256) != hipSuccess ||
+        (palettes && hipMalloc(&b->palette, palettes * 256 * 3 * sizeof(float)) != hipSuccess) ||
         hipMalloc(&b->d_table, n * sizeof(BatchSceneInfo)) != hipSuccess ||
         hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess)
         return cleanup(fail(VX_ERR_OUT_OF_MEMORY, "vx_batch_create: device allocation failed"));
-    uint64_t nb = 0, bb = 0, cb = 0;
+    uint64_t nb = 0, bb = 0, cb = 0, pb = 0;
     for (uint32_t s = 0; s < n; ++s) {
         const vx_scene* sc = scenes[s];
         if (hipMemcpy(b->nodes + nb, sc->nodes, sc->node_count * 8, hipMemcpyDeviceToDevice) != hipSuccess ||
@@ -367,7 +372,9 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
               hipMemcpy(b->occ + bb * 16, sc->occ, sc->brick_count * 64, hipMemcpyDeviceToDevice) != hipSuccess ||
               hipMemcpy(b->box + bb, sc->box, sc->brick_count * 4, hipMemcpyDeviceToDevice) != hipSuccess)) ||
             (sc->view.bc1 && sc->brick_count &&
-             hipMemcpy(b->bc1 + cb * 32, sc->bc1, sc->brick_count * 256, hipMemcpyDeviceToDevice) != hipSuccess))
+             hipMemcpy(b->bc1 + cb * 32, sc->bc1, sc->brick_count * 256, hipMemcpyDeviceToDevice) != hipSuccess) ||
+            (sc->palette && hipMemcpy(b->palette + pb * 768, sc->palette, 768 * sizeof(float),
+                                      hipMemcpyDeviceToDevice) != hipSuccess))
             return cleanup(fail(VX_ERR_HIP, "vx_batch_create: pool copy failed"));
         SceneView v = sc->view;
         v.nodes = b->nodes + nb;
@@ -375,6 +382,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         v.occ = b->occ + bb * 16;
         v.box = b->box + bb;
         v.bc1 = sc->view.bc1 ? b->bc1 + cb * 32 : nullptr;
+        v.palette = sc->palette ? b->palette + pb * 768 : nullptr;
         v.nrm = nullptr; // the batched launch writes hit records only (no shading)
         v.lds_off = 0;
         v.lds_count = 0;
@@ -383,6 +391,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
         nb += sc->node_count;
         bb += sc->brick_count;
         if (sc->view.bc1) cb += sc->brick_count;
+        if (sc->palette) ++pb;
     }
     *out = b;
     return VX_OK;
@@ -508,6 +517,7 @@ vx_status vx_batch_destroy(vx_batch* b)
     (void)hipFree(b->occ);
     (void)hipFree(b->box);
     (void)hipFree(b->bc1);
+    (void)hipFree(b->palette);
     (void)hipFree(b->d_table);
     (void)hipFree(b->d_queue);
     delete b;

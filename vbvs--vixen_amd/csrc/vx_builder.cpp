@@ -270,7 +270,8 @@ static vx_status build_tree(const std::vector<uint64_t>& codes, uint32_t N, int 
                     for (int t = 0; t < 16; ++t) {
                         if (w[t] != 0u) {
                             any = true;
-                            rgba_to_float(w[t], cols[t]);
+                            if (with_bc1 == VX_BC1_FROM_MATERIAL) vx_material_id_to_color(w[t], cols[t]);
+                            else rgba_to_float(w[t], cols[t]);
                         } else {
                             cols[t][0] = cols[t][1] = cols[t][2] = 0.0f;
                         }
@@ -316,6 +317,8 @@ static vx_status build_from_dense(const uint32_t* grid, uint32_t N, int with_bc1
                                   vx_built_scene** out)
 {
     if (!grid || !out) return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: null argument");
+    if (with_bc1 < VX_BC1_NONE || with_bc1 > VX_BC1_FROM_MATERIAL)
+        return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: with_bc1 must be VX_BC1_NONE, _FROM_RGBA8 or _FROM_MATERIAL");
     if (!valid_edge(N))
         return fail(VX_ERR_INVALID_ARG, "vx_build_from_dense: N must be a power of two in [16, 4096]");
     *out = nullptr;
@@ -360,6 +363,8 @@ static vx_status build_from_bricks(const uint32_t* cells, const uint32_t* voxels
                                    int with_bc1, int nthreads, vx_built_scene** out)
 {
     if (!out || (n && (!cells || !voxels))) return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: null argument");
+    if (with_bc1 < VX_BC1_NONE || with_bc1 > VX_BC1_FROM_MATERIAL)
+        return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: with_bc1 must be VX_BC1_NONE, _FROM_RGBA8 or _FROM_MATERIAL");
     if (!valid_edge(N))
         return fail(VX_ERR_INVALID_ARG, "vx_build_from_bricks: N must be a power of two in [16, 4096]");
     *out = nullptr;

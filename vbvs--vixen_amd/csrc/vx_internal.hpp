@@ -35,6 +35,7 @@ struct vx_scene {
     uint32_t* box; // per brick: box of its occupied voxels (see SceneView::box)
     uint2* bc1;
     uint4* nrm; // DXT normal blocks (32 per brick) or nullptr
+    float* palette = nullptr; // 256 x RGB material palette (always the scene's own copy) or nullptr
     uint64_t node_count;
     uint64_t brick_count;
     vx_octree_config cfg;

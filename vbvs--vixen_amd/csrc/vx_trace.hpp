@@ -63,6 +63,7 @@ struct SceneView {
     const uint32_t* __restrict__ box;
     const uint2* __restrict__ bc1;            // 32 blocks per brick or nullptr
     const uint4* __restrict__ nrm;            // 32 DXT normal blocks per brick or nullptr
+    const float* __restrict__ palette;        // 256 x RGB by material id (brick word & 0xFF) or nullptr
     uint32_t node_count;
     uint32_t brick_count;
     float gmin[3];
@@ -179,6 +180,16 @@ __device__ __forceinline__ void rgba8_decode(uint32_t w, float rgb[3])
     rgb[0] = (float)(w & 255u) / 255.0f;
     rgb[1] = (float)((w >> 8) & 255u) / 255.0f;
     rgb[2] = (float)((w >> 16) & 255u) / 255.0f;
+}
+
+// The uncompressed pipeline's colour of a voxel word: getMaterialColor(word & 0xFF)
+// (shaders/VoxelRayMarch.comp:189-191) as a 256-entry palette.
+__device__ __forceinline__ void palette_decode(const float* __restrict__ palette, uint32_t w, float rgb[3])
+{
+    const float* c = palette + (w & 255u) * 3u;
+    rgb[0] = c[0];
+    rgb[1] = c[1];
+    rgb[2] = c[2];
 }
 
 // Ray set-up in the grid's local voxel space [0,N]^3. Returns false on a miss
@@ -448,6 +459,7 @@ __device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
     h.voxel = *lane_word(s, s.aux_base, 4);
     if (f & VX_HIT_FLAG_LOD) h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
     else if (s.bc1) bc1_decode(ld32(s.bc1, h.brick * 32u + (h.voxel >> 4)), h.voxel & 15u, h.rgb);
+    else if (s.palette) palette_decode(s.palette, s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
     else rgba8_decode(s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
 }
 
