@@ -220,6 +220,17 @@ vx_status vx_scene_destroy(vx_scene* scene);
  * waiting for its backward pass, retains it and calls vx_scene_destroy when done. */
 vx_status vx_scene_retain(vx_scene* scene);
 
+/* Re-checks a scene whose buffers the caller changed in place, typically a
+ * vx_scene_create_device scene (no reference counterpart: VIXEN re-uploads).
+ * Copies the descriptors to the host, runs vx_scene_create's tree checks
+ * against the scene's brick count, rebuilds the derived occupancy masks and
+ * boxes from the current brick data and drops the differentiable renderer's
+ * voxel rows (rebuilt on next use). If the checks fail, every later launch on
+ * the scene returns VX_ERR_INVALID_SCENE until a revalidation succeeds.
+ * Synchronous; nothing may run on the scene meanwhile. Batches keep the copies
+ * they made at vx_batch_create. */
+vx_status vx_scene_revalidate(vx_scene* scene);
+
 /* Descriptor/brick counts and tree shape of an uploaded scene. */
 vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* brick_count,
                         int32_t* user_max_levels, int32_t* bricks_per_axis);

@@ -68,3 +68,51 @@ def test_device_scene_pools_are_checked(vx, cornell_bc1, gpu_ctx):
         vx.Scene.from_device(gpu_ctx, t["nodes"].cpu(), t["brick_data"], built.config)
     sc = vx.Scene.from_device(gpu_ctx, t["nodes"], t["brick_data"], built.config, bc1=t["bc1"])
     sc.close()
+
+
+def test_device_scene_changed_in_place_is_refused_until_revalidated(vx, orc, cornell, gpu_ctx):
+    # RGBA8 scene (no BC1) over device tensors: an in-place change blocks launches (torch's version
+    # counter); revalidate() re-checks the tree, rebuilds the occupancy tables and the trace then matches the
+    # oracle on the changed buffers. A broken tree is refused by the C ABI itself, with no launch.
+    import ctypes as C
+    import torch
+    from conftest import record_diff
+    grid, cam, built = cornell
+    nodes = torch.from_numpy(np.ascontiguousarray(built.nodes).view(np.uint8).reshape(-1).copy()).cuda()
+    bricks = torch.from_numpy(np.ascontiguousarray(built.brick_data).reshape(-1).view(np.int32).copy()).cuda()
+    sc = vx.Scene.from_device(gpu_ctx, nodes, bricks, built.config)
+    rays = orc.camera_rays(cam.as_floats(), 64, 64, 1e4)
+    d_rays = vx.rays_to_device(rays)
+    sc.trace(d_rays)
+    # empty every other brick in place
+    view = bricks.view(-1, 512)
+    view[::2] = 0
+    with pytest.raises(vx.VixenError, match="revalidate"):
+        sc.trace(d_rays)
+    sc.revalidate()
+    got = vx.hits_to_numpy(sc.trace(d_rays))
+    torch.cuda.synchronize()
+    changed = built.brick_data.copy()
+    changed[::2] = 0
+    b2 = vx.BuiltScene(built.nodes, changed, None, built.brick_coords, built.internal_count, built.config)
+    want, _ = orc.trace(orc.OracleScene.from_built(b2), rays, nthreads=8)
+    assert len(record_diff(got, want)) == 0
+    # a child pointer past the descriptor array: revalidation fails and the scene refuses launches
+    bad = built.nodes.copy()
+    root = int(bad[0, 0])
+    bad[0, 0] = (root & ~0x7FFF) | 0x7FFF if not root & 0x8000 else root
+    if root & 0x8000:
+        bad[0, 1] = 0xFFFFFF
+    nodes.copy_(torch.from_numpy(bad.view(np.uint8).reshape(-1).copy()).cuda())
+    with pytest.raises(vx.VixenError, match="VX_ERR_INVALID_SCENE"):
+        sc.revalidate()
+    hits = torch.empty_like(d_rays)
+    st = vx.LIB.vx_trace(sc._h, d_rays.data_ptr(), len(rays), hits.data_ptr(), None)
+    assert st == 2 and b"revalidate" in vx.LIB.vx_last_error()
+    # restored: accepted again
+    nodes.copy_(torch.from_numpy(np.ascontiguousarray(built.nodes).view(np.uint8).reshape(-1).copy()).cuda())
+    sc.revalidate()
+    got = vx.hits_to_numpy(sc.trace(d_rays))
+    torch.cuda.synchronize()
+    assert len(record_diff(got, want)) == 0
+    sc.close()

@@ -156,6 +156,7 @@ def _load() -> C.CDLL:
         "vx_scene_create": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_validate": ([C.POINTER(VxSceneDesc)], st),
         "vx_scene_retain": ([vp], st),
+        "vx_scene_revalidate": ([vp], st),
         "vx_scene_create_device": ([vp, C.POINTER(VxSceneDesc), C.POINTER(vp)], st),
         "vx_scene_destroy": ([vp], st),
         "vx_scene_info": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)], st),
@@ -219,7 +220,8 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_encode_normals", "vx_dxt_normal_encode_block", "vx_geometric_normal", "vx_cache_read",
             "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
             "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device",
-            "vx_albedo_backward_noalloc", "vx_material_color", "vx_material_palette", "vx_material_id_to_color")
+            "vx_albedo_backward_noalloc", "vx_material_color", "vx_material_palette", "vx_material_id_to_color",
+            "vx_scene_revalidate")
 
 
 def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:
@@ -650,6 +652,7 @@ class Scene:
         self.ctx = ctx
         self.built = built
         desc, self._cfg = _scene_desc(built)
+        self._keep, self._versions = (), ()
         h = C.c_void_p()
         _check(LIB.vx_scene_create(ctx._h, C.byref(desc), C.byref(h)), "vx_scene_create")
         self._h = h
@@ -660,7 +663,9 @@ class Scene:
         """A scene over caller-owned device buffers (vx_scene_create_device), e.g. CUDA tensors holding the
         descriptor pairs, brick words, BC1 blocks and DXT normal blocks. They are used in place: the scene keeps
         them referenced, and they must stay unchanged until close() (the kernels read them without range
-        checks; the tree was validated once, at creation). Each must be a contiguous CUDA tensor on the
+        checks; the tree was validated once, at creation). Every launch checks that no tensor was modified in
+        place since (torch's version counter) and raises otherwise; after an intended change, revalidate()
+        re-checks the tree and rebuilds the derived tables. Each must be a contiguous CUDA tensor on the
         context's device; bc1 and normals must hold 256 and 512 bytes per brick. palette (host, (256, 3)
         float32, e.g. material_palette()) is copied."""
         self = cls.__new__(cls)
@@ -680,7 +685,8 @@ class Scene:
             pal = np.ascontiguousarray(palette, dtype=np.float32)
             if pal.shape != (256, 3):
                 raise ValueError("palette: a (256, 3) float32 array of RGB per material id")
-        self._keep = (nodes, brick_data, bc1, normals)
+        self._keep = tuple(t for t in (nodes, brick_data, bc1, normals) if t is not None)
+        self._versions = tuple(t._version for t in self._keep)
         desc = VxSceneDesc()
         desc.material_palette = _ptr(pal) if pal is not None else None
         desc.esvo_nodes = nodes.data_ptr()
@@ -706,18 +712,31 @@ class Scene:
         except Exception:
             pass
 
+    def _live(self) -> C.c_void_p:
+        """The handle, after checking that no borrowed device buffer changed since creation or revalidate()."""
+        if any(t._version != v for t, v in zip(self._keep, self._versions)):
+            raise VixenError(2, "Scene", "a borrowed device buffer was modified in place since the scene was "
+                                         "validated; call revalidate() first")
+        return self._h
+
+    def revalidate(self) -> None:
+        """vx_scene_revalidate: re-checks the tree of the (possibly modified) borrowed buffers and rebuilds the
+        derived occupancy tables; launches work again once it succeeds."""
+        _check(LIB.vx_scene_revalidate(self._h), "vx_scene_revalidate")
+        self._versions = tuple(t._version for t in self._keep)
+
     # rays/hits are contiguous CUDA tensors (any dtype) on the scene's device holding n * 32 bytes
     def trace(self, rays, hits=None, stream=None):
         n = _ray_count(rays, "rays", self.ctx.device)
         hits = _hits_for(hits, n, rays, self.ctx.device)
-        _check(LIB.vx_trace(self._h, rays.data_ptr(), n, hits.data_ptr(), _stream_ptr(stream)), "vx_trace")
+        _check(LIB.vx_trace(self._live(), rays.data_ptr(), n, hits.data_ptr(), _stream_ptr(stream)), "vx_trace")
         return hits
 
     def trace_lod(self, rays, lod: VxLod, hits=None, stream=None):
         """castRayWithLOD over a batch; LOD hits carry VX_HIT_FLAG_LOD (bit 6)."""
         n = _ray_count(rays, "rays", self.ctx.device)
         hits = _hits_for(hits, n, rays, self.ctx.device)
-        _check(LIB.vx_trace_lod(self._h, rays.data_ptr(), n, C.byref(lod), hits.data_ptr(), _stream_ptr(stream)),
+        _check(LIB.vx_trace_lod(self._live(), rays.data_ptr(), n, C.byref(lod), hits.data_ptr(), _stream_ptr(stream)),
                "vx_trace_lod")
         return hits
 
@@ -726,7 +745,7 @@ class Scene:
         n = _ray_count(rays, "rays", self.ctx.device)
         hits = _hits_for(hits, n, rays, self.ctx.device)
         counters = torch.zeros(8, dtype=torch.int64, device=rays.device)
-        _check(LIB.vx_trace_counters(self._h, rays.data_ptr(), n, hits.data_ptr(), counters.data_ptr(),
+        _check(LIB.vx_trace_counters(self._live(), rays.data_ptr(), n, hits.data_ptr(), counters.data_ptr(),
                                      _stream_ptr(stream)), "vx_trace_counters")
         return hits, counters
 
@@ -735,7 +754,7 @@ class Scene:
         dev = torch.device("cuda", self.ctx.device)
         hb = torch.empty(w * h * 32, dtype=torch.uint8, device=dev) if hits else None
         ib = torch.empty(w * h, dtype=torch.int32, device=dev) if rgba else None
-        _check(LIB.vx_render(self._h, C.byref(cam), w, h, float(tmax), hb.data_ptr() if hb is not None else None,
+        _check(LIB.vx_render(self._live(), C.byref(cam), w, h, float(tmax), hb.data_ptr() if hb is not None else None,
                              ib.data_ptr() if ib is not None else None, _stream_ptr(stream)), "vx_render")
         return hb, ib
 
@@ -746,7 +765,7 @@ class Scene:
         n = _ray_count(primary, "primary", self.ctx.device)
         _dev_buf(hits, "hits", self.ctx.device, 32 * n)
         out = torch.empty(n * 32, dtype=torch.uint8, device=primary.device)
-        _check(LIB.vx_bounce_rays(self._h, primary.data_ptr(), hits.data_ptr(), n, seed & 0xFFFFFFFF, out.data_ptr(),
+        _check(LIB.vx_bounce_rays(self._live(), primary.data_ptr(), hits.data_ptr(), n, seed & 0xFFFFFFFF, out.data_ptr(),
                                   _stream_ptr(stream)), "vx_bounce_rays")
         return out
 
@@ -756,7 +775,7 @@ class Scene:
     # ---- differentiable albedo rendering (Lambertian, one RGB albedo per occupied voxel) ----
     def voxel_count(self) -> int:
         n = C.c_uint64()
-        _check(LIB.vx_scene_voxel_count(self._h, C.byref(n)), "vx_scene_voxel_count")
+        _check(LIB.vx_scene_voxel_count(self._live(), C.byref(n)), "vx_scene_voxel_count")
         return int(n.value)
 
     def albedo_forward(self, cam: VxCamera, w: int, h: int, light, albedo, store_hits: bool = True, stream=None):
@@ -770,7 +789,7 @@ class Scene:
         image = torch.empty((h, w, 3), dtype=torch.float32, device=albedo.device)
         rows = torch.empty(h * w, dtype=torch.int32, device=albedo.device) if store_hits else None
         shade = torch.empty(h * w, dtype=torch.float32, device=albedo.device) if store_hits else None
-        _check(LIB.vx_albedo_forward(self._h, C.byref(cam), w, h, lt.ctypes.data, albedo.data_ptr(), image.data_ptr(),
+        _check(LIB.vx_albedo_forward(self._live(), C.byref(cam), w, h, lt.ctypes.data, albedo.data_ptr(), image.data_ptr(),
                                      rows.data_ptr() if store_hits else None, shade.data_ptr() if store_hits else None,
                                      _stream_ptr(stream)), "vx_albedo_forward")
         return image, rows, shade
@@ -794,7 +813,7 @@ class Scene:
         else:
             _dev_buf(grad, "grad", dev, 12 * V)
         lt = np.ascontiguousarray(light, dtype=np.float32)
-        _check(LIB.vx_albedo_backward(self._h, C.byref(cam), w, h, lt.ctypes.data, grad_image.data_ptr(),
+        _check(LIB.vx_albedo_backward(self._live(), C.byref(cam), w, h, lt.ctypes.data, grad_image.data_ptr(),
                                       rows.data_ptr() if rows is not None else None,
                                       shade.data_ptr() if shade is not None else None, grad.data_ptr(),
                                       1 if deterministic else 0, _stream_ptr(stream)), "vx_albedo_backward")
@@ -813,7 +832,7 @@ class Batch:
     def __init__(self, ctx: Context, scenes: list[Scene]):
         self.ctx = ctx
         self.scenes = list(scenes)
-        arr = (C.c_void_p * len(scenes))(*[sc._h.value for sc in scenes])
+        arr = (C.c_void_p * len(scenes))(*[sc._live().value for sc in scenes])
         h = C.c_void_p()
         _check(LIB.vx_batch_create(ctx._h, arr, len(scenes), C.byref(h)), "vx_batch_create")
         self._h = h
@@ -989,7 +1008,7 @@ def albedo_render(scene: "Scene", albedo, cam: VxCamera, w: int, h: int, light, 
     # the binding runs on the current stream (its outputs come from torch's allocator there)
     with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
         s = torch.cuda.current_stream()
-        return torch_ext().albedo_render(albedo, int(scene._h.value), camt, int(w), int(h), lt, bool(store_hits),
+        return torch_ext().albedo_render(albedo, int(scene._live().value), camt, int(w), int(h), lt, bool(store_hits),
                                          bool(deterministic), int(s.cuda_stream))
 
 

@@ -504,6 +504,7 @@ vx_status vx_scene_voxel_count(vx_scene* s, uint64_t* n_voxels)
 {
     if (!s || !n_voxels) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_voxel_count: null argument");
     VX_DEVICE_GUARD(s->device);
+    if (vx_status us = vx::scene_usable(s, "vx_albedo"); us != VX_OK) return us;
     vx_status st;
     vx_albedo_state* a = vx::albedo_state(s, nullptr, st);
     if (!a) return st;
@@ -519,6 +520,7 @@ vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint3
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_albedo_forward: image too large");
     VX_DEVICE_GUARD(s->device);
+    if (vx_status us = vx::scene_usable(s, "vx_albedo"); us != VX_OK) return us;
     hipStream_t st = static_cast<hipStream_t>(stream);
     vx_status status;
     vx_albedo_state* a = vx::albedo_state(s, st, status);
@@ -555,6 +557,7 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
                                             " is " + std::to_string(px) + " pixels, more than the " +
                                             std::to_string(VX_NOALLOC_MAX_PIXELS) + " its workspace holds");
     VX_DEVICE_GUARD(s->device);
+    if (vx_status us = vx::scene_usable(s, "vx_albedo"); us != VX_OK) return us;
     vx_albedo_state* a = s->albedo;
     if (!a && noalloc)
         return fail(VX_ERR_INVALID_ARG, std::string(where) + ": the scene has no voxel row table yet (it is built, "

@@ -860,6 +860,7 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
                               hipStream_t st, const vx_lod* lod = nullptr)
 {
     if (!s || (n && (!rays || !hits))) return fail(VX_ERR_INVALID_ARG, "vx_trace: null argument");
+    if (vx_status st = scene_usable(s, "vx_trace"); st != VX_OK) return st;
     if (n == 0) return VX_OK;
     VX_DEVICE_GUARD(s->device);
     const uint64_t chunks = (n + kBlock - 1) / kBlock;
@@ -987,6 +988,44 @@ vx_status vx_scene_info(const vx_scene* s, uint64_t* node_count, uint64_t* brick
     return VX_OK;
 }
 
+vx_status vx_scene_revalidate(vx_scene* s)
+{
+    if (!s) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_revalidate: null scene");
+    VX_DEVICE_GUARD(s->device);
+    std::vector<uint32_t> host(s->node_count * 2);
+    if (hipMemcpy(host.data(), s->nodes, s->node_count * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return vx::fail(VX_ERR_HIP, "vx_scene_revalidate: reading the descriptors failed");
+    vx_scene_desc hd{};
+    hd.esvo_nodes = host.data();
+    hd.node_count = s->node_count;
+    hd.brick_data = s->brick_data;
+    hd.brick_count = s->brick_count;
+    hd.config = &s->cfg;
+    uint32_t level_prefix[4];
+    if (vx_status st = vx::scene_validate_impl(&hd, level_prefix); st != VX_OK) {
+        s->invalid.store(true, std::memory_order_release);
+        return st;
+    }
+    const uint64_t nb = s->brick_count;
+    if (nb) {
+        const uint64_t words = nb * 16;
+        vx::k_build_occ<<<dim3((unsigned)((words + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0, 0>>>(
+            s->brick_data, words, s->occ);
+        vx::k_build_box<<<dim3((unsigned)((nb + vx::kBlock - 1) / vx::kBlock)), dim3(vx::kBlock), 0, 0>>>(s->occ, nb,
+                                                                                                           s->box);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            s->invalid.store(true, std::memory_order_release);
+            return vx::fail(VX_ERR_HIP, "vx_scene_revalidate: occupancy mask build failed");
+        }
+    }
+    std::memcpy(s->level_prefix, level_prefix, sizeof level_prefix);
+    // the differentiable renderer's voxel rows come from the occupancy: rebuilt on next use
+    vx_albedo_state_free(s->albedo);
+    s->albedo = nullptr;
+    s->invalid.store(false, std::memory_order_release);
+    return VX_OK;
+}
+
 vx_status vx_scene_set_lds_depth(vx_scene* s, uint32_t depth)
 {
     if (!s || depth > 4) return vx::fail(VX_ERR_INVALID_ARG, "vx_scene_set_lds_depth: depth must be 0..4");
@@ -1040,6 +1079,7 @@ vx_status vx_render(vx_scene* s, const vx_camera* cam, uint32_t w, uint32_t h, f
     if (!s || !cam || (!hits && !rgba8)) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: null argument");
     if (w == 0 || h == 0) return VX_OK;
     if (w > 65536 || h > 65536) return vx::fail(VX_ERR_INVALID_ARG, "vx_render: image too large");
+    if (vx_status st = vx::scene_usable(s, "vx_render"); st != VX_OK) return st;
     VX_DEVICE_GUARD(s->device);
     const vx::CamParams c = vx::make_cam(cam);
     const uint32_t tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16, ntiles = tiles_x * tiles_y;

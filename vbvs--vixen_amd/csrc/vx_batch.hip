@@ -327,6 +327,7 @@ static vx_status batch_create(vx_ctx* ctx, vx_scene* const* scenes, uint32_t n, 
     uint64_t nodes = 0, bricks = 0, bc1_bricks = 0, palettes = 0;
     for (uint32_t s = 0; s < n; ++s) {
         if (!scenes[s]) return fail(VX_ERR_INVALID_ARG, "vx_batch_create: null scene");
+        if (vx_status st = scene_usable(scenes[s], "vx_batch_create"); st != VX_OK) return st;
         if (scenes[s]->device != ctx->device)
             return fail(VX_ERR_INVALID_ARG, "vx_batch_create: scene " + std::to_string(s) + " lives on device " +
                                                 std::to_string(scenes[s]->device) + ", the context on " +

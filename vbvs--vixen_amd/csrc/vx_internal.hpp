@@ -51,6 +51,8 @@ struct vx_scene {
     // references: the creator's, plus one per vx_scene_retain; vx_scene_destroy
     // drops one and the last frees the scene
     std::atomic<int> refs{1};
+    // set when a vx_scene_revalidate finds the buffers broken: launches refuse the scene
+    std::atomic<bool> invalid{false};
 };
 
 namespace vx {
@@ -95,6 +97,13 @@ private:
     if (vx_device_guard_.error() != hipSuccess)                                                       \
         return ::vx::fail(VX_ERR_HIP, std::string("could not make device ") + std::to_string(device) + \
                                           " current: " + hipGetErrorString(vx_device_guard_.error()))
+
+// VX_OK, or the error every launch on a scene that failed vx_scene_revalidate returns.
+inline vx_status scene_usable(const vx_scene* s, const char* who)
+{
+    if (!s->invalid.load(std::memory_order_acquire)) return VX_OK;
+    return fail(VX_ERR_INVALID_SCENE, std::string(who) + ": the scene failed its last vx_scene_revalidate");
+}
 
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kMaxLdsBytes = 65536;
