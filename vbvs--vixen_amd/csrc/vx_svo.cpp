@@ -252,12 +252,13 @@ LaineKarrasOctreeHip::LaineKarrasOctreeHip(int device, const vx_scene_desc& d) :
 
 LaineKarrasOctreeHip::~LaineKarrasOctreeHip()
 {
-    {
-        OnDevice on(device_);
-        if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
-        (void)hipFree(d_rays_);
-        (void)hipFree(d_hits_);
-    }
+    // no exceptions here: the device is switched by hand, and every call's status is dropped
+    int prev = -1;
+    const bool switched = hipGetDevice(&prev) == hipSuccess && prev != device_ && hipSetDevice(device_) == hipSuccess;
+    if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
+    (void)hipFree(d_rays_);
+    (void)hipFree(d_hits_);
+    if (switched) (void)hipSetDevice(prev);
     (void)vx_scene_destroy(scene_);
     (void)vx_ctx_destroy(ctx_);
 }
@@ -276,13 +277,13 @@ void LaineKarrasOctreeHip::trace_host(const vx_ray* rays, size_t n, const LODPar
 {
     std::lock_guard<std::mutex> lock(mu_);
     OnDevice on(device_);
-    if (!d_rays_) {
+    if (!stream_) {
         hipStream_t st = nullptr;
         check_hip(hipStreamCreate(&st), "hipStreamCreate");
         stream_ = st;
-        check_hip(hipMalloc(reinterpret_cast<void**>(&d_rays_), kStage * sizeof(vx_ray)), "hipMalloc");
-        check_hip(hipMalloc(reinterpret_cast<void**>(&d_hits_), kStage * sizeof(vx_hit)), "hipMalloc");
     }
+    if (!d_rays_) check_hip(hipMalloc(reinterpret_cast<void**>(&d_rays_), kStage * sizeof(vx_ray)), "hipMalloc");
+    if (!d_hits_) check_hip(hipMalloc(reinterpret_cast<void**>(&d_hits_), kStage * sizeof(vx_hit)), "hipMalloc");
     auto st = static_cast<hipStream_t>(stream_);
     for (size_t at = 0; at < n; at += kStage) {
         const size_t m = std::min(kStage, n - at);
