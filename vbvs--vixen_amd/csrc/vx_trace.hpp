@@ -494,10 +494,13 @@ inline uint32_t lay_out_lanes(SceneView& v, uint32_t staged, uint32_t lanes, boo
 struct RayState {
     float tc[3], tb[3];      // ESVO coefficients
     float pos[3];
-    float scale_exp2, t_min, t_max;
+    float t_min, t_max;
     uint32_t parent;
     int idx, scale, iter;
     int mask;                // octant mask (bits 0-2) | can-step flags (bits 3-5)
+    // scale_exp2 = 2^(scale - 23), derived from scale in one instruction rather
+    // than carried: one loop-carried register less to copy between branches
+    __device__ __forceinline__ float scale_exp2() const { return u2f(((uint32_t)scale + 104u) << 23); }
 };
 
 // Ray set-up, coefficients and initTraversalState. Returns true when the ray
@@ -547,7 +550,6 @@ __device__ __forceinline__ bool init_ray(const SceneView& s, const float o[3], c
 
     r.parent = 0u;
     r.scale = ESVO_MAX_SCALE;
-    r.scale_exp2 = 0.5f;
     r.iter = 0;
     int idx = 0;
     const bool use_pos = t_min < 1e-4f;
@@ -609,7 +611,7 @@ __device__ __forceinline__ void brick_cell(const SceneView& s, const RayState& r
     const int octm = r.mask & 7;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2 - r.pos[i]);
+        const float lp = (octm & (1 << i)) ? r.pos[i] : (3.0f - r.scale_exp2() - r.pos[i]);
         bc[i] = (int)((lp - 1.0f) * s.Bf);
     }
 }
@@ -620,13 +622,14 @@ __device__ __forceinline__ void brick_cell(const SceneView& s, const RayState& r
 __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, float tx, float ty, float tz, float tcm)
 {
     const int iter = r.iter;
+    const float sc = r.scale_exp2();
     const bool canx = r.mask & 8, cany = r.mask & 16, canz = r.mask & 32;
     const float tca =
         tcm >= 1e10f ? fmx(fmx(canx ? tx : -1e10f, cany ? ty : -1e10f), canz ? tz : -1e10f) : tcm;
     int step_mask = 0;
-    if (canx && tx <= tca) { step_mask ^= 1; r.pos[0] -= r.scale_exp2; }
-    if (cany && ty <= tca) { step_mask ^= 2; r.pos[1] -= r.scale_exp2; }
-    if (canz && tz <= tca) { step_mask ^= 4; r.pos[2] -= r.scale_exp2; }
+    if (canx && tx <= tca) { step_mask ^= 1; r.pos[0] -= sc; }
+    if (cany && ty <= tca) { step_mask ^= 2; r.pos[1] -= sc; }
+    if (canz && tz <= tca) { step_mask ^= 4; r.pos[2] -= sc; }
     r.t_min = fmx(tca, 0.0f); // tca: finite, never -0 (see corrected_tc_max_f)
     r.idx ^= step_mask;
     r.iter = iter + 1;
@@ -646,16 +649,15 @@ __device__ __forceinline__ bool advance_ray(const SceneView& s, RayState& r, flo
         return false;
     }
     uint32_t diff = 0u;
-    if (step_mask & 1) diff |= f2u(r.pos[0]) ^ f2u(r.pos[0] + r.scale_exp2);
-    if (step_mask & 2) diff |= f2u(r.pos[1]) ^ f2u(r.pos[1] + r.scale_exp2);
-    if (step_mask & 4) diff |= f2u(r.pos[2]) ^ f2u(r.pos[2] + r.scale_exp2);
+    if (step_mask & 1) diff |= f2u(r.pos[0]) ^ f2u(r.pos[0] + sc);
+    if (step_mask & 2) diff |= f2u(r.pos[1]) ^ f2u(r.pos[1] + sc);
+    if (step_mask & 4) diff |= f2u(r.pos[2]) ^ f2u(r.pos[2] + sc);
     if (diff == 0u) {
         rec_miss(s, (uint32_t)(iter + 1));
         return true;
     }
     const int scale = (int)((f2u((float)diff) >> 23) - 127u);
     r.scale = scale;
-    r.scale_exp2 = u2f((uint32_t)(scale - ESVO_MAX_SCALE - 1 + 127) << 23);
     if (scale < s.min_scale || scale > ESVO_MAX_SCALE) {
         rec_miss(s, (uint32_t)(iter + 1));
         return true;
@@ -736,7 +738,8 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
     const float tz = r.pos[2] * r.tc[2] - r.tb[2];
     const float tcm = corrected_tc_max_f(tx, ty, tz, r.mask, r.t_max);
     const float tv_max = fmn(r.t_max, tcm); // finite, never -0 (see corrected_tc_max_f)
-    const float half = r.scale_exp2 * 0.5f;
+    const float sc = r.scale_exp2();
+    const float half = sc * 0.5f;
     const float cx = half * r.tc[0] + tx;
     const float cy = half * r.tc[1] + ty;
     const float cz = half * r.tc[2] + tz;
@@ -744,7 +747,7 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
     if (LOD && process) {
         const float dist = u2f(*lane_word(s, s.aux_base, 9)) + r.t_min * s.lod_ray_length;
         // the child's edge is scale_exp2 = 2^(scale - 23) of the unit cube (esvoScaleToWorldSize)
-        if (dist * s.lod_dir + s.lod_orig >= r.scale_exp2 * s.lod_world_size) {
+        if (dist * s.lod_dir + s.lod_orig >= sc * s.lod_world_size) {
             // a coarse hit at this child: the reference's placeholder normal (0,1,0),
             // its user level (esvoToUserScale) in bits 8-12, its parent descriptor
             rec_hit(s,
@@ -778,10 +781,9 @@ __device__ __forceinline__ int esvo_step(const SceneView& s, RayState& r, TraceC
             r.parent = cp + popc8(valid & ~leaf & below);
             int idx = 0;
             r.scale -= 1;
-            r.scale_exp2 = half;
-            if (cx > r.t_min) { idx ^= 1; r.pos[0] += r.scale_exp2; }
-            if (cy > r.t_min) { idx ^= 2; r.pos[1] += r.scale_exp2; }
-            if (cz > r.t_min) { idx ^= 4; r.pos[2] += r.scale_exp2; }
+            if (cx > r.t_min) { idx ^= 1; r.pos[0] += half; }
+            if (cy > r.t_min) { idx ^= 2; r.pos[1] += half; }
+            if (cz > r.t_min) { idx ^= 4; r.pos[2] += half; }
             r.idx = idx;
             r.t_max = tv_max;
             r.iter = iter + 1;
