@@ -14,19 +14,18 @@
 // Hits are either read back from the forward pass (8 bytes per pixel) or
 // recomputed by re-tracing (no per-pixel storage). Two reductions:
 //   atomic         one FP32 atomic add per hit pixel and channel;
-//   deterministic  (row, pixel) pairs are radix-sorted by row (stable, so
-//                  equal rows keep pixel order, misses last); the run heads
-//                  are selected (integer work) and each run is summed in a
-//                  fixed order (one thread in sequence for runs of <= 32 pairs,
-//                  else one workgroup: thread t takes elements t, t+256, ...,
-//                  then a fixed tree), so the per-row sums, added without
-//                  conflicts, depend only on the input: identical bits on
-//                  every run. (A decoupled look-back reduce-by-key is not
-//                  used: the order in which it combines the partial sums of a
-//                  run spanning several tiles depends on timing.)
+//   deterministic  (row, pixel) pairs are sorted by row with a stable LSD
+//                  radix sort written here (equal rows keep pixel order,
+//                  misses last); a run head sums its run in a fixed order (one
+//                  thread in sequence for runs of <= 32 pairs, else one
+//                  workgroup: thread t takes elements t, t+256, ..., then a
+//                  fixed tree), so the per-row sums, added without conflicts,
+//                  depend only on the input: identical bits on every run. (A
+//                  decoupled look-back reduce-by-key is not used: the order in
+//                  which it combines the partial sums of a run spanning several
+//                  tiles depends on timing.)
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -40,11 +39,6 @@ namespace vx {
 
 constexpr uint32_t kAlbBlock = 256;
 
-// LSD radix sort (onesweep) at every size: rocprim's default switches to a
-// block-sort + merge-pass algorithm below 2^20 items, which at one 1280x720
-// view costs about 20 launches instead of 3 onesweep passes over the key bits.
-using PairSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                  rocprim::default_config, 0>;
 constexpr uint32_t kNoVoxel = 0xFFFFFFFFu;
 
 struct Rgb {
@@ -218,81 +212,361 @@ __global__ void __launch_bounds__(kAlbBlock)
     }
 }
 
-// Run heads of the sorted keys: index i starts a run when its key differs from
-// the previous one.
-struct RunHead {
-    const uint32_t* keys;
-    __host__ __device__ bool operator()(uint32_t i) const { return i == 0u || keys[i] != keys[i - 1u]; }
-};
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) pairs, 8 key bits per pass. A pass is
+// three launches: k_sort_count (each tile of kSortTile pairs counts its
+// digits), k_sort_scan (one workgroup per digit: exclusive scan of that digit's
+// counts over the tiles, and the digit's total) and k_sort_scatter (each tile
+// writes its pairs to digit base + earlier tiles' count + rank in the tile).
+// A tile is kSortWaves contiguous slices of kSortRounds x 64 pairs, one per
+// wave; a wave ranks its slice round by round, matching digits across its lanes
+// (one ballot per key bit) and keeping its own digit counters in LDS, so pairs
+// are ranked in index order without atomics: the sort is stable, and every
+// count is an integer, so the output order does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortWaves = kSortThreads / 64;
+constexpr uint32_t kSortRounds = 8; // pairs per thread and tile
+constexpr uint32_t kSortTile = kSortThreads * kSortRounds;
+constexpr uint32_t kRadix = 256;
+static_assert(kSortThreads == kRadix, "one thread per digit in the scan and bookkeeping steps");
 
-// Fixed-order run sums. Every run's sum is a function of its elements and
-// length only, so the gradient bits do not depend on scheduling. Runs of at
-// most kShortRun pairs: one thread adds them in order. Longer runs (voxels that
-// cover many pixels) are listed for k_reduce_long_runs. The last run, of
-// misses (key miss_key), is skipped. Rows are unique across runs: no conflicts.
-constexpr uint32_t kShortRun = 32;
-
-__device__ __forceinline__ uint32_t run_end(const uint32_t* heads, uint32_t r, uint32_t n, uint32_t n_items)
+// Lanes among `active` whose 8-bit digit equals this lane's (every lane of the
+// wave executes it).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active)
 {
-    return r + 1u < n ? heads[r + 1u] : n_items;
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t ones = __ballot(bit);
+        m &= bit ? ones : ~ones;
+    }
+    return m;
 }
 
-__global__ void __launch_bounds__(kAlbBlock) k_reduce_short_runs(const uint32_t* __restrict__ keys_sorted,
-                                                                const Rgb* __restrict__ vals,
-                                                                const uint32_t* __restrict__ pix_sorted,
-                                                                const uint32_t* __restrict__ heads,
-                                                                uint32_t* __restrict__ counts, uint32_t n_items,
-                                                                uint32_t miss_key, float* __restrict__ grad,
-                                                                uint32_t* __restrict__ long_runs)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m, uint32_t lane)
 {
-    const uint32_t n = counts[0];
-    for (uint32_t r = blockIdx.x * kAlbBlock + threadIdx.x; r < n; r += gridDim.x * kAlbBlock) {
-        const uint32_t b = heads[r], e = run_end(heads, r, n, n_items);
-        const uint32_t row = keys_sorted[b];
-        if (row >= miss_key) continue;
-        if (e - b > kShortRun) {
-            long_runs[atomicAdd(&counts[1], 1u)] = r; // list order is irrelevant to the sums
-            continue;
-        }
-        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-        for (uint32_t i = b; i < e; ++i) {
-            const Rgb v = vals[pix_sorted[i]];
-            sr += v.r;
-            sg += v.g;
-            sb += v.b;
-        }
-        grad[(size_t)row * 3 + 0] += sr;
-        grad[(size_t)row * 3 + 1] += sg;
-        grad[(size_t)row * 3 + 2] += sb;
+    return (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Exclusive scan of one value per thread over a workgroup of kSortThreads;
+// `total` receives the sum of all values. Integer adds: the order is irrelevant.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total)
+{
+    __shared__ uint32_t wsum[kSortWaves];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63u) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; ++w) {
+        before += w < wave ? wsum[w] : 0u;
+        all += wsum[w];
+    }
+    __syncthreads(); // wsum is reused by the next call
+    total = all;
+    return before + x - v;
+}
+
+// Digits of the wave's slice of the tile, counted into its LDS counters c[]
+// (zeroed by the caller); rank[k] receives each pair's rank among the slice's
+// pairs of its digit. The wave's LDS reads and writes execute in order, so a
+// round's reads see the previous round's counts.
+__device__ __forceinline__ void rank_slice(const uint32_t (&d)[kSortRounds], uint32_t base, uint32_t n,
+                                           uint32_t* c, uint32_t (&rank)[kSortRounds])
+{
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortRounds; ++k) {
+        const bool act = base + k * 64u + lane < n;
+        const uint64_t peers = match_digit(d[k], __ballot(act));
+        const uint32_t below = lanes_below(peers, lane);
+        const uint32_t before = act ? c[d[k]] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (act && below == 0u) c[d[k]] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rank[k] = before + below;
     }
 }
 
-// One workgroup per listed long run: thread t adds elements t, t+256, t+512,
-// ... in order into two interleaved partial sums (independent loads in flight),
-// then the threads' sums are combined by a fixed tree in LDS.
+__global__ void __launch_bounds__(kSortThreads) k_sort_count(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             uint32_t shift, uint32_t n_tiles,
+                                                             uint32_t* __restrict__ count)
+{
+    __shared__ uint32_t c[kSortWaves][kRadix];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; ++w) c[w][t] = 0u;
+    const uint32_t base = blockIdx.x * kSortTile + wave * (kSortRounds * 64u); // this wave's slice
+    uint32_t d[kSortRounds], rank[kSortRounds];
+#pragma unroll
+    for (uint32_t k = 0; k < kSortRounds; ++k) { // every load issued before any is used
+        const uint32_t i = base + k * 64u + lane;
+        d[k] = i < n ? (keys[i] >> shift) & 0xFFu : 0u;
+    }
+    __syncthreads();
+    rank_slice(d, base, n, c[wave], rank);
+    __syncthreads();
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; ++w) sum += c[w][t];
+    count[t * n_tiles + blockIdx.x] = sum;
+}
+
+// Workgroup d: exclusive scan of count[d][0 .. n_tiles) in place, total[d] = the
+// digit's count.
+__global__ void __launch_bounds__(kSortThreads) k_sort_scan(uint32_t* __restrict__ count, uint32_t n_tiles,
+                                                            uint32_t* __restrict__ total)
+{
+    uint32_t* c = count + (size_t)blockIdx.x * n_tiles;
+    uint32_t running = 0;
+    for (uint32_t b = 0; b < n_tiles; b += kSortThreads) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < n_tiles ? c[i] : 0u;
+        uint32_t sum;
+        const uint32_t ex = block_exclusive_scan(v, sum);
+        if (i < n_tiles) c[i] = running + ex;
+        running += sum;
+    }
+    if (threadIdx.x == 0u) total[blockIdx.x] = running;
+}
+
+// GATHER (the last pass): the values are pixel indices, and each pair's
+// contribution contrib[pixel] is also written to sorted[pos], so the run sums
+// read contributions in sorted order instead of gathering them one by one.
+template <bool GATHER>
+__global__ void __launch_bounds__(kSortThreads)
+    k_sort_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n,
+                   uint32_t shift, uint32_t n_tiles, const uint32_t* __restrict__ count,
+                   const uint32_t* __restrict__ total, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                   const Rgb* __restrict__ contrib, Rgb* __restrict__ sorted)
+{
+    __shared__ uint32_t c[kSortWaves][kRadix]; // per-wave digit counts, then each wave's first slot per digit
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; ++w) c[w][t] = 0u;
+    const uint32_t base = blockIdx.x * kSortTile + wave * (kSortRounds * 64u);
+    uint32_t key[kSortRounds], val[kSortRounds], d[kSortRounds], rank[kSortRounds];
+#pragma unroll
+    for (uint32_t k = 0; k < kSortRounds; ++k) {
+        const uint32_t i = base + k * 64u + lane;
+        key[k] = i < n ? keys_in[i] : 0u;
+        val[k] = i < n ? vals_in[i] : 0u;
+    }
+    Rgb g[GATHER ? kSortRounds : 1];
+    if (GATHER) {
+#pragma unroll
+        for (uint32_t k = 0; k < kSortRounds; ++k) // in flight while the pairs are ranked
+            if (base + k * 64u + lane < n) g[k] = contrib[val[k]];
+    }
+    uint32_t all;
+    const uint32_t tile_base = block_exclusive_scan(total[t], all) + count[t * n_tiles + blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < kSortRounds; ++k) d[k] = (key[k] >> shift) & 0xFFu;
+    rank_slice(d, base, n, c[wave], rank);
+    __syncthreads();
+    // digit t: wave w's pairs start after the tile's earlier waves' pairs of that digit
+    uint32_t first = tile_base;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; ++w) {
+        const uint32_t cw = c[w][t];
+        c[w][t] = first;
+        first += cw;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kSortRounds; ++k) {
+        if (base + k * 64u + lane < n) {
+            const uint32_t pos = c[wave][d[k]] + rank[k];
+            keys_out[pos] = key[k];
+            vals_out[pos] = val[k];
+            if (GATHER) sorted[pos] = g[k];
+        }
+    }
+}
+
+// Fixed-order run sums over the sorted pairs. Every run's sum is a function of
+// its elements and length only, so the gradient bits do not depend on
+// scheduling. Each workgroup takes a window of kRunWindow positions, kRunItems
+// consecutive ones per thread: it marks the run starts (key differs from the
+// previous one) in an LDS bit mask and finds each run's end at the next start;
+// a run still open at the window's end is either closed within the next
+// kShortRun positions (independent loads) or certainly long. The runs of at
+// most kShortRun pairs are compacted into an LDS list and added in order, one
+// thread per run. Longer runs (voxels that cover many pixels) go to the
+// window's kLongPerWindow slots of the long-run list, with their end, or with
+// end 0 when it lies past the window, for k_reduce_long_runs (no global
+// counter: a single atomic counter shared by every window serialised badly on
+// views with many long runs). Misses (key miss_key) sort last and are skipped.
+// Rows are unique across runs: no conflicts.
+constexpr uint32_t kShortRun = 32;
+constexpr uint32_t kRunItems = 2;
+constexpr uint32_t kRunWindow = kAlbBlock * kRunItems;
+static_assert(kAlbBlock == kSortThreads, "k_reduce_runs uses block_exclusive_scan");
+static_assert(32u % kRunItems == 0u && kRunWindow / 32u <= kAlbBlock, "a thread's run starts fit in one mask word");
+// long runs start at least kShortRun + 1 positions apart
+constexpr uint32_t kLongPerWindow = kRunWindow / (kShortRun + 1u) + 1u;
+
+__global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __restrict__ keys_sorted,
+                                                          const Rgb* __restrict__ vals_sorted, uint32_t n,
+                                                          uint32_t miss_key, float* __restrict__ grad,
+                                                          uint2* __restrict__ long_runs, uint32_t* __restrict__ long_count)
+{
+    __shared__ uint32_t starts[kRunWindow / 32]; // bit i: window position i starts a run
+    __shared__ uint2 runs[kRunWindow];           // short runs: first position, end
+    __shared__ uint32_t run_row[kRunWindow];
+    const uint32_t t = threadIdx.x;
+    const uint32_t w0 = blockIdx.x * kRunWindow, p0 = w0 + t * kRunItems;
+    uint32_t row[kRunItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) row[j] = p0 + j < n ? keys_sorted[p0 + j] : 0u;
+    const uint32_t before = (p0 > 0u && p0 - 1u < n) ? keys_sorted[p0 - 1u] : ~row[0];
+    uint32_t mine = 0; // this thread's run starts, bit j = position p0 + j
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        const uint32_t prev = j == 0u ? before : row[j - 1u];
+        if (p0 + j < n && prev != row[j]) mine |= 1u << j;
+    }
+    if (t < kRunWindow / 32u) starts[t] = 0u;
+    __syncthreads();
+    atomicOr(&starts[(t * kRunItems) >> 5], mine << ((t * kRunItems) & 31u)); // OR: order-free
+    __syncthreads();
+    uint32_t end[kRunItems];
+    uint32_t n_short = 0, n_longs = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        end[j] = 0u;
+        if (!((mine >> j) & 1u)) continue;
+        // the next run start after window position L, else past the window
+        const uint32_t L = t * kRunItems + j;
+        uint32_t wi = L >> 5, bits = (L & 31u) == 31u ? 0u : starts[wi] & (~0u << ((L & 31u) + 1u));
+        while (bits == 0u && ++wi < kRunWindow / 32u) bits = starts[wi];
+        if (row[j] >= miss_key) continue; // the misses' run
+        const uint32_t p = p0 + j, wend = w0 + kRunWindow;
+        uint32_t e = 0u; // 0: past the window and longer than kShortRun
+        if (bits != 0u) {
+            e = w0 + wi * 32u + (uint32_t)(__ffs(bits) - 1);
+        } else if (wend >= n) {
+            e = n;
+        } else if (wend - p <= kShortRun) {
+            // positions p .. wend - 1 are in the run: is it closed within kShortRun + 1 pairs?
+            uint32_t differ = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kShortRun; ++k) {
+                const uint32_t q = wend + k;
+                if (q < p + kShortRun + 1u && (q >= n || keys_sorted[q < n ? q : 0u] != row[j])) differ |= 1u << k;
+            }
+            if (differ != 0u) e = wend + (uint32_t)(__ffs(differ) - 1);
+            e = e < n ? e : n;
+        }
+        end[j] = e;
+        if (e == 0u || e - p > kShortRun) ++n_longs;
+        else ++n_short;
+    }
+    uint32_t total, total_long;
+    uint32_t slot = block_exclusive_scan(n_short, total);
+    uint32_t lslot = block_exclusive_scan(n_longs, total_long);
+    uint2* my_long = long_runs + (size_t)blockIdx.x * kLongPerWindow;
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        const uint32_t p = p0 + j;
+        if (!((mine >> j) & 1u) || row[j] >= miss_key) continue;
+        if (end[j] != 0u && end[j] - p <= kShortRun) {
+            runs[slot] = make_uint2(p, end[j]);
+            run_row[slot] = row[j];
+            ++slot;
+        } else {
+            my_long[lslot++] = make_uint2(p, end[j]);
+        }
+    }
+    if (threadIdx.x == 0u) long_count[blockIdx.x] = total_long;
+    __syncthreads();
+    for (uint32_t r = t; r < total; r += kAlbBlock) {
+        const uint2 run = runs[r];
+        // added in order, four loads in flight at a time
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        for (uint32_t i = run.x; i < run.y; i += 4u) {
+            Rgb v[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k)
+                if (i + k < run.y) v[k] = vals_sorted[i + k];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k)
+                if (i + k < run.y) {
+                    sr += v[k].r;
+                    sg += v[k].g;
+                    sb += v[k].b;
+                }
+        }
+        const size_t g = (size_t)run_row[r] * 3;
+        grad[g + 0] += sr;
+        grad[g + 1] += sg;
+        grad[g + 2] += sb;
+    }
+}
+
+// One workgroup per window of k_reduce_runs, taking the long runs [b, e) listed
+// in the window's slots in turn: an end listed as 0 lies past the window, and
+// the group finds it by bisection with one probe per thread (keys are sorted:
+// the probes that still hold the row are a prefix). Thread t then adds elements
+// t, t+256, t+512, ... in order into two interleaved partial sums, loading four
+// steps ahead, and the threads' sums are combined by a fixed tree in LDS.
 __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* __restrict__ keys_sorted,
-                                                               const Rgb* __restrict__ vals,
-                                                               const uint32_t* __restrict__ pix_sorted,
-                                                               const uint32_t* __restrict__ heads,
-                                                               const uint32_t* __restrict__ counts, uint32_t n_items,
-                                                               const uint32_t* __restrict__ long_runs,
+                                                               const Rgb* __restrict__ vals_sorted, uint32_t n,
+                                                               const uint2* __restrict__ long_runs,
+                                                               const uint32_t* __restrict__ long_count,
                                                                float* __restrict__ grad)
 {
     __shared__ float part[3][kAlbBlock];
+    __shared__ uint32_t probe[kAlbBlock];
     const uint32_t t = threadIdx.x;
-    const uint32_t n = counts[0], n_long = counts[1];
-    for (uint32_t j = blockIdx.x; j < n_long; j += gridDim.x) {
-        const uint32_t r = long_runs[j]; // the same j, so the same run, in the whole group
-        const uint32_t b = heads[r], e = run_end(heads, r, n, n_items);
+    const uint32_t count = long_count[blockIdx.x];
+    for (uint32_t j = 0; j < count; ++j) {
+        const uint2 run = long_runs[(size_t)blockIdx.x * kLongPerWindow + j];
+        const uint32_t b = run.x;
+        const uint32_t row = keys_sorted[b];
+        uint32_t e = run.y;
+        if (e == 0u) {
+            // keys[lo - 1] == row; hi == n or keys[hi] != row; the end is in [lo, hi]
+            uint32_t lo = b + 1u, hi = n;
+            while (lo < hi) {
+                const uint32_t q = lo + (uint32_t)(((uint64_t)(hi - lo) * t) / kAlbBlock);
+                probe[t] = q;
+                const int held = __syncthreads_count(keys_sorted[q] == row); // probes 0 .. held-1 hold the row
+                const uint32_t nlo = held > 0 ? probe[held - 1] + 1u : lo;
+                const uint32_t nhi = held < (int)kAlbBlock ? probe[held] : hi;
+                __syncthreads(); // probe[] is rewritten by the next round
+                lo = nlo;
+                hi = nhi;
+            }
+            e = lo;
+        }
         float s0[3] = {0.0f, 0.0f, 0.0f}, s1[3] = {0.0f, 0.0f, 0.0f};
         uint32_t i = b + t;
+        for (; i + 7u * kAlbBlock < e; i += 8u * kAlbBlock) { // four steps: eight loads in flight
+            Rgb v[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) v[k] = vals_sorted[i + k * kAlbBlock];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k += 2u) {
+                s0[0] += v[k].r; s0[1] += v[k].g; s0[2] += v[k].b;
+                s1[0] += v[k + 1].r; s1[1] += v[k + 1].g; s1[2] += v[k + 1].b;
+            }
+        }
         for (; i + kAlbBlock < e; i += 2u * kAlbBlock) {
-            const Rgb v0 = vals[pix_sorted[i]], v1 = vals[pix_sorted[i + kAlbBlock]];
+            const Rgb v0 = vals_sorted[i], v1 = vals_sorted[i + kAlbBlock];
             s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
             s1[0] += v1.r; s1[1] += v1.g; s1[2] += v1.b;
         }
         if (i < e) {
-            const Rgb v0 = vals[pix_sorted[i]];
+            const Rgb v0 = vals_sorted[i];
             s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
         }
         part[0][t] = s0[0] + s1[0];
@@ -308,26 +582,30 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* 
             __syncthreads();
         }
         if (t == 0u) {
-            const uint32_t row = keys_sorted[b];
             grad[(size_t)row * 3 + 0] += part[0][0];
             grad[(size_t)row * 3 + 1] += part[1][0];
             grad[(size_t)row * 3 + 2] += part[2][0];
         }
-        __syncthreads(); // part[] is reused by the next run
+        __syncthreads(); // part[] and probe[] are reused by the next run
     }
 }
 
-__global__ void k_clear_long_count(uint32_t* counts) { counts[1] = 0u; }
-
 // Device buffers of the deterministic reduction for up to cap_px pixels.
 struct DetWorkspace {
-    uint32_t *keys, *keys_sorted, *heads;
-    uint32_t* n_runs; // [0] runs, [1] long runs
-    Rgb* vals;        // contributions, by pixel
-    uint32_t *pix, *pix_sorted;
-    void* tmp;        // sort and run-selection temporary storage
-    size_t cap_px, cap_tmp;
+    uint32_t *keys, *keys_sorted; // row keys: written in pixel order, sorted in place of each other
+    uint32_t *pix, *pix_sorted;   // the pixel of each key
+    Rgb* vals;                    // contributions, by pixel
+    Rgb* vals_sorted;             // contributions in sorted order
+    uint2* long_runs;             // [first, end) of each run longer than kShortRun, kLongPerWindow slots per window
+    uint32_t* long_count;         // slots used per window
+    uint32_t* count;              // the sort's digit counts per tile (kRadix x tiles), then its kRadix digit totals
+    size_t cap_px;
 };
+
+__host__ __device__ constexpr size_t sort_tiles(size_t px) { return (px + kSortTile - 1) / kSortTile; }
+// words of DetWorkspace::count for px pixels
+constexpr size_t count_words(size_t px) { return kRadix * sort_tiles(px) + kRadix; }
+constexpr size_t run_windows(size_t px) { return (px + kRunWindow - 1) / kRunWindow; }
 
 // Workspace of vx_albedo_backward_noalloc: one array per device, kept until the
 // process ends, shared by every call on that device (calls that overlap on one
@@ -419,37 +697,28 @@ static SceneView alb_view(const vx_scene* s, uint32_t& lds)
     return v;
 }
 
-static vx_status grow(vx_albedo_state* a, size_t px, size_t tmp)
+static vx_status grow(vx_albedo_state* a, size_t px)
 {
     DetWorkspace& w = a->ws;
-    if (px > w.cap_px) {
-        for (void* p : {(void*)w.keys, (void*)w.keys_sorted, (void*)w.heads, (void*)w.n_runs, (void*)w.vals,
-                        (void*)w.pix, (void*)w.pix_sorted})
-            (void)hipFree(p);
-        w.keys = w.keys_sorted = w.heads = w.n_runs = w.pix = w.pix_sorted = nullptr;
-        w.vals = nullptr;
-        w.cap_px = 0;
-        if (hipMalloc(&w.keys, px * 4) != hipSuccess || hipMalloc(&w.keys_sorted, px * 4) != hipSuccess ||
-            hipMalloc(&w.heads, px * 4) != hipSuccess || hipMalloc(&w.n_runs, 8) != hipSuccess ||
-            hipMalloc(&w.vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&w.pix, px * 4) != hipSuccess ||
-            hipMalloc(&w.pix_sorted, px * 4) != hipSuccess)
-            return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
-        w.cap_px = px;
-    }
-    if (tmp > w.cap_tmp) {
-        (void)hipFree(w.tmp);
-        w.tmp = nullptr;
-        w.cap_tmp = 0;
-        if (hipMalloc(&w.tmp, tmp) != hipSuccess) return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
-        w.cap_tmp = tmp;
-    }
+    if (px <= w.cap_px) return VX_OK;
+    for (void* p : {(void*)w.keys, (void*)w.keys_sorted, (void*)w.pix, (void*)w.pix_sorted, (void*)w.vals,
+                    (void*)w.vals_sorted, (void*)w.long_runs, (void*)w.long_count, (void*)w.count})
+        (void)hipFree(p);
+    w = DetWorkspace{};
+    if (hipMalloc(&w.keys, px * 4) != hipSuccess || hipMalloc(&w.keys_sorted, px * 4) != hipSuccess ||
+        hipMalloc(&w.pix, px * 4) != hipSuccess || hipMalloc(&w.pix_sorted, px * 4) != hipSuccess ||
+        hipMalloc(&w.vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&w.vals_sorted, px * sizeof(Rgb)) != hipSuccess ||
+        hipMalloc(&w.long_runs, run_windows(px) * kLongPerWindow * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&w.long_count, run_windows(px) * 4) != hipSuccess ||
+        hipMalloc(&w.count, count_words(px) * 4) != hipSuccess)
+        return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
+    w.cap_px = px;
     return VX_OK;
 }
 
 // The deterministic reduction's workspace for px pixels carved from the
-// device's scratch array: 256-byte aligned slices, the temporary storage last.
-// Fails when px pixels and tmp bytes do not fit.
-static vx_status scratch_workspace(int device, size_t px, size_t tmp, DetWorkspace& w)
+// device's scratch array in 256-byte aligned slices. Fails when they do not fit.
+static vx_status scratch_workspace(int device, size_t px, DetWorkspace& w)
 {
     void* base = nullptr;
     {
@@ -471,14 +740,14 @@ static vx_status scratch_workspace(int device, size_t px, size_t tmp, DetWorkspa
     };
     w.keys = static_cast<uint32_t*>(take(px * 4));
     w.keys_sorted = static_cast<uint32_t*>(take(px * 4));
-    w.heads = static_cast<uint32_t*>(take(px * 4));
-    w.n_runs = static_cast<uint32_t*>(take(8));
-    w.vals = static_cast<Rgb*>(take(px * sizeof(Rgb)));
     w.pix = static_cast<uint32_t*>(take(px * 4));
     w.pix_sorted = static_cast<uint32_t*>(take(px * 4));
-    w.tmp = take(tmp);
+    w.vals = static_cast<Rgb*>(take(px * sizeof(Rgb)));
+    w.vals_sorted = static_cast<Rgb*>(take(px * sizeof(Rgb)));
+    w.long_runs = static_cast<uint2*>(take(run_windows(px) * kLongPerWindow * sizeof(uint2)));
+    w.long_count = static_cast<uint32_t*>(take(run_windows(px) * 4));
+    w.count = static_cast<uint32_t*>(take(count_words(px) * 4));
     w.cap_px = px;
-    w.cap_tmp = tmp;
     if (off > kScratchBytes)
         return fail(VX_ERR_INVALID_ARG, "vx_albedo_backward_noalloc: " + std::to_string(px) + " pixels need " +
                                             std::to_string(off) + " bytes of workspace, the scratch array holds " +
@@ -492,8 +761,8 @@ void vx_albedo_state_free(vx_albedo_state* a)
 {
     if (!a) return;
     const vx::DetWorkspace& w = a->ws;
-    for (void* p : {(void*)a->base, (void*)w.keys, (void*)w.keys_sorted, (void*)w.heads, (void*)w.n_runs,
-                    (void*)w.vals, (void*)w.pix, (void*)w.pix_sorted, w.tmp})
+    for (void* p : {(void*)a->base, (void*)w.keys, (void*)w.keys_sorted, (void*)w.pix, (void*)w.pix_sorted,
+                    (void*)w.vals, (void*)w.vals_sorted, (void*)w.long_runs, (void*)w.long_count, (void*)w.count})
         (void)hipFree(p);
     delete a;
 }
@@ -581,26 +850,18 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
         if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
         return VX_OK;
     }
-    // deterministic: sort (row, contribution) by row, select the run heads, sum each run in a fixed order
+    // deterministic: sort (row, pixel) by row, then sum each run in a fixed order
     if (px > 0x7FFFFFFFu) return fail(VX_ERR_INVALID_ARG, std::string(where) + ": deterministic mode needs w*h < 2^31");
     if (a->voxels >= 0xFFFFFFFFull) return fail(VX_ERR_UNSUPPORTED, std::string(where) + ": too many voxels");
     // misses get key V (the voxel count) and sort last; the sort looks at the key bits V needs
     const uint32_t miss_key = (uint32_t)a->voxels;
-    int end_bit = 1;
+    uint32_t end_bit = 1;
     while (end_bit < 32 && (miss_key >> end_bit) != 0u) ++end_bit;
-    size_t sort_bytes = 0, red_bytes = 0;
-    const hipcub::CountingInputIterator<uint32_t> index(0u);
-    if (rocprim::radix_sort_pairs<PairSortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)px, 0u,
-                                                  (unsigned)end_bit, st) != hipSuccess ||
-        hipcub::DeviceSelect::If(nullptr, red_bytes, index, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)px,
-                                 RunHead{nullptr}, st) != hipSuccess)
-        return fail(VX_ERR_HIP, std::string(where) + ": workspace query failed");
     DetWorkspace ws{};
     if (noalloc) {
-        if (vx_status g = scratch_workspace(s->device, px, std::max(sort_bytes, red_bytes), ws); g != VX_OK) return g;
+        if (vx_status g = scratch_workspace(s->device, px, ws); g != VX_OK) return g;
     } else {
-        if (vx_status g = grow(a, px, std::max(sort_bytes, red_bytes)); g != VX_OK) return g;
+        if (vx_status g = grow(a, px); g != VX_OK) return g;
         ws = a->ws;
     }
     if (recompute)
@@ -610,23 +871,30 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
         k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
                                                                nullptr, ws.keys, ws.vals, ws.pix, miss_key);
     if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
-    size_t tb = ws.cap_tmp;
-    if (rocprim::radix_sort_pairs<PairSortConfig>(ws.tmp, tb, ws.keys, ws.keys_sorted, ws.pix, ws.pix_sorted,
-                                                  (uint32_t)px, 0u, (unsigned)end_bit, st) != hipSuccess)
-        return fail(VX_ERR_HIP, std::string(where) + ": sort failed");
-    tb = ws.cap_tmp;
-    if (hipcub::DeviceSelect::If(ws.tmp, tb, index, ws.heads, ws.n_runs, (int)px, RunHead{ws.keys_sorted}, st) !=
-        hipSuccess)
-        return fail(VX_ERR_HIP, std::string(where) + ": run selection failed");
-    // resident groups walk all runs (their count stays on the device); the keys
-    // buffer, consumed by the sort, holds the list of long runs
-    const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->cu_count * 8u,
-                                                                               (px + kAlbBlock - 1) / kAlbBlock));
-    k_clear_long_count<<<1, 1, 0, st>>>(ws.n_runs);
-    k_reduce_short_runs<<<dim3(groups), block, 0, st>>>(ws.keys_sorted, ws.vals, ws.pix_sorted, ws.heads, ws.n_runs,
-                                                        (uint32_t)px, miss_key, grad_albedo, ws.keys);
-    k_reduce_long_runs<<<dim3(groups), block, 0, st>>>(ws.keys_sorted, ws.vals, ws.pix_sorted, ws.heads, ws.n_runs,
-                                                       (uint32_t)px, ws.keys, grad_albedo);
+    // 8 key bits per pass, ping-ponging between the two key and pixel buffers
+    const uint32_t n = (uint32_t)px, n_tiles = (uint32_t)sort_tiles(px);
+    uint32_t *kin = ws.keys, *vin = ws.pix, *kout = ws.keys_sorted, *vout = ws.pix_sorted;
+    uint32_t* total = ws.count + (size_t)kRadix * n_tiles;
+    for (uint32_t shift = 0; shift < end_bit; shift += 8) {
+        k_sort_count<<<dim3(n_tiles), dim3(kSortThreads), 0, st>>>(kin, n, shift, n_tiles, ws.count);
+        k_sort_scan<<<dim3(kRadix), dim3(kSortThreads), 0, st>>>(ws.count, n_tiles, total);
+        if (shift + 8u >= end_bit) // the last pass also puts the contributions in sorted order
+            k_sort_scatter<true><<<dim3(n_tiles), dim3(kSortThreads), 0, st>>>(kin, vin, n, shift, n_tiles, ws.count,
+                                                                               total, kout, vout, ws.vals,
+                                                                               ws.vals_sorted);
+        else
+            k_sort_scatter<false><<<dim3(n_tiles), dim3(kSortThreads), 0, st>>>(kin, vin, n, shift, n_tiles, ws.count,
+                                                                                total, kout, vout, nullptr, nullptr);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": sort failed");
+    // the sorted keys are in kin, their contributions in vals_sorted; one workgroup per window of positions
+    const uint32_t n_windows = (uint32_t)run_windows(px);
+    k_reduce_runs<<<dim3(n_windows), block, 0, st>>>(kin, ws.vals_sorted, n, miss_key, grad_albedo, ws.long_runs,
+                                                     ws.long_count);
+    k_reduce_long_runs<<<dim3(n_windows), block, 0, st>>>(kin, ws.vals_sorted, n, ws.long_runs, ws.long_count,
+                                                          grad_albedo);
     if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": run reduction failed");
     return VX_OK;
 }
