@@ -123,6 +123,45 @@ def test_validate_rejects_corrupt_trees(vx, cornell):
     assert st == VX_ERR_INVALID_SCENE and "brick" in msg
 
 
+def test_validate_survives_seeded_corruption(vx, cornell):
+    """Seeded mutations of a built tree (valid and leaf masks, far bits, child pointers, brick ids, node
+    counts): validation answers every one with a status (run with the library built under
+    AddressSanitizer this also checks that the walk reads only descriptors that exist), and a tree it
+    accepts has every reachable child block inside the array and every reachable brick id valid, which
+    is what the traversal kernels rely on."""
+    built = cornell[2]
+    rng = np.random.default_rng(5)
+    outcomes = set()
+    for i in range(400):
+        nodes = built.nodes.copy()
+        nn = nodes.shape[0]
+        for _ in range(int(rng.integers(1, 6))):
+            k, w = int(rng.integers(0, nn)), int(rng.integers(0, 2))
+            if i % 2:
+                nodes[k, w] ^= np.uint32(1 << int(rng.integers(0, 32)))
+            else:
+                nodes[k, w] = np.uint32(rng.integers(0, 2 ** 32))
+        count = int(rng.integers(1, nn + 1)) if i % 7 == 0 else nn
+        st, _ = _validate(vx, built, nodes=nodes, node_count=count)
+        outcomes.add(st)
+        assert st in (0, 2), st
+        if st == 0:  # the accepted tree, walked from the root as the kernels walk it
+            todo = [0]
+            while todo:
+                idx = todo.pop()
+                w0, w1 = int(nodes[idx, 0]), int(nodes[idx, 1])
+                valid, leaf = (w0 >> 16) & 0xFF, (w0 >> 24) & 0xFF
+                cp = (w1 & 0xFFFFFF) if w0 & 0x8000 else (w0 & 0x7FFF)
+                for c in range(bin(valid).count("1")):
+                    assert cp + c < count
+                    if leaf:
+                        b = int(nodes[cp + c, 1]) & 0xFFFFFF
+                        assert b == 0xFFFFFF or b < built.brick_count
+                    else:
+                        todo.append(cp + c)
+    assert outcomes == {0, 2}
+
+
 def test_validate_rejects_bad_configs(vx, cornell):
     built = cornell[2]
     for field, value in (("bricks_per_axis", 3), ("bricks_per_axis", 1), ("user_max_levels", 7),

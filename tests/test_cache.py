@@ -154,3 +154,44 @@ def test_error_cases(vx, scenes, tmp_path):
     (tmp_path / "count.vsca").write_bytes(_file_bytes([bytes(e)]))
     st, msg = _status(vx, tmp_path / "count.vsca")
     assert st == 2 and "sizes" in msg
+
+
+def test_corrupted_files_fail_cleanly(vx, scenes, tmp_path):
+    """Seeded corruption of a valid two-entry file: 4-byte words set to hostile values (0, 1, 2^31-1,
+    2^32-1, random; counts and sizes among them), random byte flips, truncation anywhere. Every file
+    gets a status (never a crash: run with the library built under AddressSanitizer this also checks
+    that the reader reads only the bytes it holds), and a file the reader accepts yields entries whose
+    scene descriptions vx_scene_validate accepts or refuses, again without a crash (their descriptor and
+    brick arrays are not checked by the reader, only their sizes)."""
+    cave, cornell, mats = scenes
+    good = _file_bytes([_entry_bytes(vx, cave, 2, 32, 0.5, 9), _entry_bytes(vx, cornell, 1, 64, 0.1, 1, mats)])
+    rng = np.random.default_rng(77)
+    path = tmp_path / "fuzz.vsca"
+    seen = {0: 0, 2: 0, 5: 0}
+    for i in range(300):
+        b = bytearray(good)
+        if i % 3 == 0:
+            for _ in range(int(rng.integers(1, 4))):
+                off = int(rng.integers(0, len(b) - 4)) & ~3
+                val = [0, 1, 0x7FFFFFFF, 0xFFFFFFFF, int(rng.integers(0, 2 ** 32))][int(rng.integers(0, 5))]
+                b[off:off + 4] = struct.pack("<I", val)
+        elif i % 3 == 1:
+            for _ in range(int(rng.integers(1, 16))):
+                b[int(rng.integers(0, len(b)))] ^= int(rng.integers(1, 256))
+        else:
+            b = b[:int(rng.integers(0, len(b)))]
+        path.write_bytes(bytes(b))
+        h = C.c_void_p()
+        st = vx.LIB.vx_cache_read(str(path).encode(), C.byref(h))
+        assert st in seen, (i, st, vx.LIB.vx_last_error())
+        seen[st] += 1
+        if st != 0:
+            continue
+        n = C.c_uint32()
+        assert vx.LIB.vx_cache_entry_count(h, C.byref(n)) == 0
+        for e in range(n.value):
+            info, desc = vx.VxCacheEntryInfo(), vx.VxSceneDesc()
+            assert vx.LIB.vx_cache_entry(h, e, C.byref(info), C.byref(desc)) == 0
+            vx.LIB.vx_scene_validate(C.byref(desc))  # any status; it must not read out of bounds
+        vx.LIB.vx_cache_free(h)
+    assert seen[0] > 0 and seen[2] > 0  # both outcomes occurred
