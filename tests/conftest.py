@@ -127,3 +127,29 @@ def aimed_rays(ray_dtype, rng, n, N, solid_xyz):
     r["d"] = tgt - r["o"]
     r["tmax"] = 3.0e38
     return r
+
+
+@pytest.fixture(scope="session")
+def brick_row(vx):
+    """A 4096^3 world holding one row of 512 bricks along x (y, z in [0, 8)). Bricks 0-510 hold one voxel
+    at (0, 7, 7), beside the line y = z = 1.5; the last brick holds voxel (7, 1, 1), on it. A ray along
+    +x on that line visits every brick it crosses, one ESVO iteration or more each, so from far enough
+    back it reaches the reference's 512-iteration cap before the last brick."""
+    N = 4096
+    nb = N // 8
+    cells = np.zeros((nb, 3), np.uint32)
+    cells[:, 0] = np.arange(nb)
+    vox = np.zeros((nb, 8, 8, 8), np.uint32)
+    vox[:-1, 7, 7, 0] = 0xFF112233  # voxels[i, z, y, x]
+    vox[-1, 1, 1, 7] = 0xFF445566
+    return N, vx.build_from_bricks(cells, vox, N, bc1=False)
+
+
+def row_rays(ray_dtype, N, starts):
+    """Rays along +x on the line y = z = 1.5 from the given x."""
+    r = np.zeros(len(starts), ray_dtype)
+    r["o"][:, 0] = starts
+    r["o"][:, 1:] = 1.5
+    r["d"][:, 0] = 1.0
+    r["tmax"] = 1e30
+    return r

@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import record_diff
+from conftest import record_diff, row_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -146,4 +146,31 @@ def test_scene_and_batch_outlive_their_context(vx, orc, edge_scenes):
     _check(got, want, "scene after its context")
     _check(vx.hits_to_numpy(hits), want, "batch after its context")
     batch.close()
+    scene.close()
+
+
+def test_iteration_cap_bit_exact(vx, orc, brick_row, gpu_ctx):
+    """Rays that end at the 512-iteration cap, and rays just under it, through each form of vx_trace: a
+    few rays (one launch, leaf visits postponed), 64 rays sharing one origin (a lockstep block) and 1.2 M
+    rays (the two-launch form, the cap rays among lane-refilled incoherent ones); records bit-exact vs
+    the oracle, the capped ones misses with iteration count 512."""
+    N, built = brick_row
+    scene = vx.Scene(gpu_ctx, built)
+    oscene = orc.OracleScene.from_built(built)
+    rng = np.random.default_rng(12)
+    few = row_rays(orc.RAY_DTYPE, N, np.array([-1.0, 100.0, N - 2000.0, N - 200.0, N - 3000.0], np.float32))
+    same = row_rays(orc.RAY_DTYPE, N, np.full(64, -1.0, np.float32))
+    same["d"][:, 1:] = rng.uniform(-1e-4, 1e-4, (64, 2))  # stays in the row for most of it
+    big = np.zeros(1_200_000, orc.RAY_DTYPE)
+    big["o"] = rng.uniform(0, N, (len(big), 3))
+    big["o"][:, 1:] = rng.uniform(0, 8, (len(big), 2))
+    big["d"] = rng.normal(size=(len(big), 3))
+    big["tmax"] = 1e30
+    k = rng.choice(len(big), 4000, replace=False)
+    big[k] = row_rays(orc.RAY_DTYPE, N, rng.uniform(-1, N - 100, len(k)).astype(np.float32))
+    for label, rays in (("few", few), ("lockstep", same), ("two-launch", big)):
+        want, _ = orc.trace(oscene, rays, nthreads=8)
+        got = vx.hits_to_numpy(scene.trace(vx.rays_to_device(rays)))
+        _check(got, want, label)
+        assert ((want["flags"] >> 16) == 512).any(), label
     scene.close()

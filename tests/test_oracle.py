@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import global_voxels
+from conftest import global_voxels, row_rays
 
 F32 = np.float32
 
@@ -391,3 +391,23 @@ def test_bruteforce_pinned_by_slab_test_per_voxel(vx, orc, N, occ):
     hit = clear & (vox_sl[:, 0] >= 0)
     assert hit.sum() > 200
     assert np.allclose(t_aw[hit], t_sl[hit], rtol=1e-12, atol=1e-12)
+
+
+def test_iteration_cap_ends_a_long_traversal(orc, brick_row):
+    """The reference loop stops after 512 iterations (shaders/ESVOTraversal.glsl:25) and the ray is a miss.
+    Along the brick row, rays that start close enough to the last brick reach its voxel (index 7 + 8 + 64)
+    at the closed-form distance 4095 - x0 with fewer iterations the closer they start; rays from further
+    back end at exactly 512 iterations with a miss record."""
+    N, built = brick_row
+    scene = orc.OracleScene.from_built(built)
+    starts = np.array([N - 200.0, N - 800.0, N - 2000.0, N - 3000.0, 100.0, -1.0], np.float32)
+    hits, _ = orc.trace(scene, row_rays(orc.RAY_DTYPE, N, starts))
+    iters = hits["flags"] >> 16
+    for i in range(3):
+        assert hits["flags"][i] & 1 and hits["brick"][i] == 511 and hits["voxel"][i] == 79
+        assert hits["t"][i] == np.float32(4095.0) - starts[i]
+        assert iters[i] < 512
+    assert iters[0] < iters[1] < iters[2]
+    for i in range(3, 6):
+        assert not hits["flags"][i] & 1 and iters[i] == 512
+        assert hits["brick"][i] == 0xFFFFFFFF and np.isinf(hits["t"][i])
