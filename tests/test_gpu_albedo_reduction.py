@@ -88,3 +88,43 @@ def test_fixed_order_small_and_degenerate(vx, cornell, gpu_ctx, case):
     got = grad.cpu().numpy()
     assert got.tobytes() == want.tobytes()
     scene.close()
+
+
+def test_stored_rows_out_of_range_count_as_misses(vx, cornell, gpu_ctx):
+    """Stored rows at or above the voxel count (a corrupt or foreign hit_row buffer) count as misses in both
+    reductions: no gradient index leaves the array. Deterministic: bit-exact against the restated order with
+    those pixels as misses. Atomic: within float32 rounding of it, and rows no valid pixel hit keep their
+    starting values exactly."""
+    import torch
+    grid, cam, built = cornell
+    scene = vx.Scene(gpu_ctx, built)
+    V = scene.voxel_count()
+    rng = np.random.default_rng(9)
+    w, h = 320, 200
+    px = w * h
+    rows = rng.integers(0, min(V, 3000), px).astype(np.uint32)
+    bad = rng.random(px) < 0.2
+    rows[bad] = rng.choice(np.array([V, V + 1, 2 ** 31, 0xFFFFFFFE], np.uint32), int(bad.sum()))
+    shade = rng.random(px, dtype=np.float32)
+    gimg = rng.standard_normal((h, w, 3), dtype=np.float32)
+    grad0 = rng.standard_normal((V, 3), dtype=np.float32)
+    want = restated_gradient(np.where(bad, np.uint32(MISS), rows), shade, gimg, grad0)
+    d_rows = torch.from_numpy(rows.view(np.int32)).cuda()
+    d_shade, d_gimg = torch.from_numpy(shade).cuda(), torch.from_numpy(gimg).cuda()
+    light = np.array([0.0, 1.0, 0.0], np.float32)
+    for mode in (1, 0):
+        grad = torch.from_numpy(grad0).cuda()
+        st = vx.LIB.vx_albedo_backward(scene._h, C.byref(cam), w, h, light.ctypes.data, d_gimg.data_ptr(),
+                                       d_rows.data_ptr(), d_shade.data_ptr(), grad.data_ptr(), mode,
+                                       torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        vx._check(st, "vx_albedo_backward")
+        got = grad.cpu().numpy()
+        if mode == 1:
+            assert got.tobytes() == want.tobytes()
+        else:
+            untouched = np.ones(V, bool)
+            untouched[rows[~bad]] = False
+            assert got[untouched].tobytes() == grad0[untouched].tobytes()
+            assert np.allclose(got, want, rtol=1e-5, atol=1e-5)
+    scene.close()

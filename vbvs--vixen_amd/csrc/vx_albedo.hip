@@ -156,13 +156,16 @@ __device__ __forceinline__ float group_sum(float v, uint64_t group, uint32_t lan
 // less rounding in the atomic unit); once the next group is a single lane the
 // remaining lanes add directly. Otherwise it emits (row, contribution) pairs
 // for the sorted reduction. Every lane runs to the end (no early exit), so the
-// cross-lane operations see the whole wave.
+// cross-lane operations see the whole wave. n_voxels is the scene's voxel
+// count: a stored row at or above it (the miss marker, or anything else the
+// caller's buffer holds there) counts as a miss, so no index leaves the
+// gradient; misses get the sort key n_voxels, which sorts after every row.
 template <bool RECOMPUTE, bool ATOMIC>
 __global__ void __launch_bounds__(kAlbBlock)
     k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
                      const float* __restrict__ grad_image, const uint32_t* __restrict__ hit_row,
                      const float* __restrict__ hit_shade, float* __restrict__ grad, uint32_t* __restrict__ keys,
-                     Rgb* __restrict__ vals, uint32_t* __restrict__ pix, uint32_t miss_key)
+                     Rgb* __restrict__ vals, uint32_t* __restrict__ pix, uint32_t n_voxels)
 {
     uint32_t px, py;
     const bool inside = tile_pixel(w, h, px, py);
@@ -174,6 +177,7 @@ __global__ void __launch_bounds__(kAlbBlock)
             row = trace_pixel(s, c, base, px, py, w, h, shade);
         } else {
             row = hit_row[p];
+            if (row >= n_voxels) row = kNoVoxel;
             shade = hit_shade[p];
         }
     }
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(kAlbBlock)
             atomicAdd(&grad[(size_t)row * 3 + 2], gb);
         }
     } else if (inside) {
-        keys[p] = row == kNoVoxel ? miss_key : row; // misses sort last
+        keys[p] = row == kNoVoxel ? n_voxels : row; // misses sort last
         vals[p] = Rgb{gr, gg, gb};
         pix[p] = (uint32_t)p;
     }
@@ -840,21 +844,22 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
     const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
     const bool recompute = hit_row == nullptr;
     const dim3 grid(tiles), block(kAlbBlock);
+    if (a->voxels >= 0xFFFFFFFFull) return fail(VX_ERR_UNSUPPORTED, std::string(where) + ": too many voxels");
+    const uint32_t n_voxels = (uint32_t)a->voxels;
     if (mode == VX_GRAD_ATOMIC) {
         if (recompute)
             k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
-                                                                   grad_albedo, nullptr, nullptr, nullptr, 0u);
+                                                                   grad_albedo, nullptr, nullptr, nullptr, n_voxels);
         else
             k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
-                                                                  grad_albedo, nullptr, nullptr, nullptr, 0u);
+                                                                  grad_albedo, nullptr, nullptr, nullptr, n_voxels);
         if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
         return VX_OK;
     }
     // deterministic: sort (row, pixel) by row, then sum each run in a fixed order
     if (px > 0x7FFFFFFFu) return fail(VX_ERR_INVALID_ARG, std::string(where) + ": deterministic mode needs w*h < 2^31");
-    if (a->voxels >= 0xFFFFFFFFull) return fail(VX_ERR_UNSUPPORTED, std::string(where) + ": too many voxels");
     // misses get key V (the voxel count) and sort last; the sort looks at the key bits V needs
-    const uint32_t miss_key = (uint32_t)a->voxels;
+    const uint32_t miss_key = n_voxels;
     uint32_t end_bit = 1;
     while (end_bit < 32 && (miss_key >> end_bit) != 0u) ++end_bit;
     DetWorkspace ws{};
