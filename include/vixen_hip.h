@@ -273,7 +273,8 @@ vx_status vx_camera_rays(const vx_camera* camera, uint32_t width, uint32_t heigh
  * workload's secondary rays; no reference counterpart): origin on the hit
  * voxel's entry face, 1e-3 voxel off the surface along its normal, direction
  * cosine-distributed about the normal from a hash of (seed, ray index). A
- * missed primary gets a zero direction (a miss when traced). n device rays. */
+ * missed primary, or a hit record whose face-axis field (flags bits 1-2) reads 3,
+ * gets a zero direction (a miss when traced). n device rays. */
 vx_status vx_bounce_rays(const vx_scene* scene, const vx_ray* primary, const vx_hit* hits, uint64_t n,
                          uint32_t seed, vx_ray* out, vx_stream stream);
 

@@ -8,7 +8,10 @@ primary hits):
   * every origin lies 1e-3 voxel off the hit voxel's entry face, on its normal's
     side, within the face's extent;
   * a missed primary gets a zero direction; the same seed gives the same rays and
-    another seed other rays.
+    another seed other rays;
+  * a hit record whose face-axis field reads 3 (the trace writes 0-2; a corrupt
+    or foreign buffer) is treated as a miss, and valid records beside it are
+    unchanged.
 """
 from __future__ import annotations
 
@@ -94,3 +97,22 @@ def test_misses_and_seeds(bounce):
     assert (b["d"][miss] == 0).all()
     assert np.array_equal(b.view(np.uint8), again)
     assert not np.array_equal(b.view(np.uint8), other)
+
+
+def test_corrupt_face_axis_counts_as_miss(vx, cave, gpu_ctx, bounce):
+    import torch
+    grid, cam, built = cave
+    _, prim, hits, b, _, _ = bounce
+    scene = vx.Scene(gpu_ctx, built)
+    bad_hits = hits.copy()
+    rng = np.random.default_rng(3)
+    corrupt = rng.random(len(bad_hits)) < 0.1
+    bad_hits["flags"][corrupt] |= np.uint32(0x7)                 # hit bit set, axis field 3
+    out = scene.bounce_rays(torch.from_numpy(prim.view(np.uint8).reshape(-1).copy()).cuda(),
+                            torch.from_numpy(bad_hits.view(np.uint8).reshape(-1).copy()).cuda(), seed=7)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(vx.RAY_DTYPE)
+    assert (got["d"][corrupt] == 0).all()
+    assert np.array_equal(got["o"][corrupt], prim["o"][corrupt])
+    assert np.array_equal(got[~corrupt].view(np.uint8), b[~corrupt].view(np.uint8))
+    scene.close()

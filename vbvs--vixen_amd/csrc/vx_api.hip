@@ -531,7 +531,9 @@ __global__ void __launch_bounds__(kBlock) k_bounce_rays(const vx_ray* __restrict
     load_ray(prim, i, o, d, tmin, tmax);
     const vx_hit h = hits[i];
     float4* dst = reinterpret_cast<float4*>(out + i);
-    if (!(h.flags & 1u)) {
+    // a miss, or a record whose face axis is not 0-2 (not one the trace writes),
+    // gives a zero-direction ray, which the trace treats as invalid
+    if (!(h.flags & 1u) || ((h.flags >> 1) & 3u) == 3u) {
         dst[0] = make_float4(o[0], o[1], o[2], 0.0f);
         dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
