@@ -63,14 +63,21 @@ __global__ void __launch_bounds__(kAlbBlock) k_brick_popc(const uint32_t* __rest
 }
 
 // Row of the hit voxel: brick base plus occupied voxels before `lin` in the brick.
+// The brick's 16 mask words (one 64-byte line) are read with four independent
+// 16-byte loads and counted with per-word masks, rather than a loop of dependent
+// loads up to the voxel's word.
 __device__ __forceinline__ uint32_t voxel_row(const SceneView& s, const uint32_t* __restrict__ base, uint32_t brick,
                                               uint32_t lin)
 {
-    const uint32_t* m = s.occ + (size_t)brick * 16u;
-    const uint32_t wq = lin >> 5;
+    const uint4* m = reinterpret_cast<const uint4*>(s.occ + (size_t)brick * 16u);
+    const uint4 q0 = m[0], q1 = m[1], q2 = m[2], q3 = m[3];
+    const uint32_t words[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    const uint32_t wq = lin >> 5, low = (1u << (lin & 31u)) - 1u;
     uint32_t r = base[brick];
-    for (uint32_t w = 0; w < wq; ++w) r += __builtin_popcount(m[w]);
-    return r + __builtin_popcount(m[wq] & ((1u << (lin & 31u)) - 1u));
+#pragma unroll
+    for (uint32_t w = 0; w < 16u; ++w) r += __builtin_popcount(words[w] & (w < wq ? ~0u : (w == wq ? low : 0u)));
+    return r;
 }
 
 __device__ __forceinline__ float face_shade(uint32_t flags, const float light[3])
@@ -105,23 +112,23 @@ __device__ __forceinline__ uint32_t trace_pixel(const SceneView& s, const AlbCam
     return voxel_row(s, base, hr.brick, hr.voxel);
 }
 
-// Pixel of lane threadIdx.x in 16x16 tile blockIdx.x (row-major tiles, 8x8 per wave).
-__device__ __forceinline__ bool tile_pixel(uint32_t w, uint32_t h, uint32_t& px, uint32_t& py)
+// Pixel of lane threadIdx.x in 16x16 tile `tile` (row-major tiles, 8x8 per wave).
+__device__ __forceinline__ bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint32_t& py)
 {
     const uint32_t tiles_x = (w + 15u) / 16u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    px = (blockIdx.x % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
-    py = (blockIdx.x / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
-    return px < w && py < h;
+    px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    return tile < tiles_x * ((h + 15u) / 16u) && px < w && py < h;
 }
 
 __global__ void __launch_bounds__(kAlbBlock)
-    k_albedo_forward(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
+    k_albedo_forward(SceneView s, AlbCam c, uint32_t w, uint32_t h, uint32_t per_xcd, const uint32_t* __restrict__ base,
                      const float* __restrict__ albedo, float* __restrict__ image, uint32_t* __restrict__ hit_row,
                      float* __restrict__ hit_shade)
 {
     uint32_t px, py;
-    if (!tile_pixel(w, h, px, py)) return;
+    if (!tile_pixel(w, h, xcd_stripe_chunk<kRenderStripe>(per_xcd), px, py)) return;
     float shade;
     const uint32_t row = trace_pixel(s, c, base, px, py, w, h, shade);
     const uint64_t p = (uint64_t)py * w + px;
@@ -162,13 +169,13 @@ __device__ __forceinline__ float group_sum(float v, uint64_t group, uint32_t lan
 // gradient; misses get the sort key n_voxels, which sorts after every row.
 template <bool RECOMPUTE, bool ATOMIC>
 __global__ void __launch_bounds__(kAlbBlock)
-    k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, const uint32_t* __restrict__ base,
+    k_albedo_adjoint(SceneView s, AlbCam c, uint32_t w, uint32_t h, uint32_t per_xcd, const uint32_t* __restrict__ base,
                      const float* __restrict__ grad_image, const uint32_t* __restrict__ hit_row,
                      const float* __restrict__ hit_shade, float* __restrict__ grad, uint32_t* __restrict__ keys,
                      Rgb* __restrict__ vals, uint32_t* __restrict__ pix, uint32_t n_voxels)
 {
     uint32_t px, py;
-    const bool inside = tile_pixel(w, h, px, py);
+    const bool inside = tile_pixel(w, h, xcd_stripe_chunk<kRenderStripe>(per_xcd), px, py);
     const uint64_t p = (uint64_t)py * w + px;
     uint32_t row = kNoVoxel;
     float shade = 0.0f;
@@ -801,9 +808,9 @@ vx_status vx_albedo_forward(vx_scene* s, const vx_camera* cam, uint32_t w, uint3
     uint32_t lds = 0;
     const vx::SceneView v = vx::alb_view(s, lds);
     const vx::AlbCam c = vx::make_alb_cam(cam, light);
-    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
-    vx::k_albedo_forward<<<dim3(tiles), dim3(vx::kAlbBlock), lds, st>>>(v, c, w, h, a->base, albedo, image, hit_row,
-                                                                        hit_shade);
+    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16), per = (tiles + vx::kXcds - 1) / vx::kXcds;
+    vx::k_albedo_forward<<<dim3(per * vx::kXcds), dim3(vx::kAlbBlock), lds, st>>>(v, c, w, h, per, a->base, albedo,
+                                                                                  image, hit_row, hit_shade);
     if (hipGetLastError() != hipSuccess) return vx::fail(VX_ERR_HIP, "vx_albedo_forward: launch failed");
     return VX_OK;
 }
@@ -841,17 +848,17 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
     uint32_t lds = 0;
     const SceneView v = alb_view(s, lds);
     const AlbCam c = make_alb_cam(cam, light);
-    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16), per = (tiles + kXcds - 1) / kXcds;
     const bool recompute = hit_row == nullptr;
-    const dim3 grid(tiles), block(kAlbBlock);
+    const dim3 grid(per * kXcds), block(kAlbBlock);
     if (a->voxels >= 0xFFFFFFFFull) return fail(VX_ERR_UNSUPPORTED, std::string(where) + ": too many voxels");
     const uint32_t n_voxels = (uint32_t)a->voxels;
     if (mode == VX_GRAD_ATOMIC) {
         if (recompute)
-            k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
+            k_albedo_adjoint<true, true><<<grid, block, lds, st>>>(v, c, w, h, per, a->base, grad_image, nullptr, nullptr,
                                                                    grad_albedo, nullptr, nullptr, nullptr, n_voxels);
         else
-            k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
+            k_albedo_adjoint<false, true><<<grid, block, 0, st>>>(v, c, w, h, per, a->base, grad_image, hit_row, hit_shade,
                                                                   grad_albedo, nullptr, nullptr, nullptr, n_voxels);
         if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
         return VX_OK;
@@ -870,10 +877,10 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
         ws = a->ws;
     }
     if (recompute)
-        k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, a->base, grad_image, nullptr, nullptr,
+        k_albedo_adjoint<true, false><<<grid, block, lds, st>>>(v, c, w, h, per, a->base, grad_image, nullptr, nullptr,
                                                                 nullptr, ws.keys, ws.vals, ws.pix, miss_key);
     else
-        k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, a->base, grad_image, hit_row, hit_shade,
+        k_albedo_adjoint<false, false><<<grid, block, 0, st>>>(v, c, w, h, per, a->base, grad_image, hit_row, hit_shade,
                                                                nullptr, ws.keys, ws.vals, ws.pix, miss_key);
     if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": launch failed");
     // 8 key bits per pass, ping-ponging between the two key and pixel buffers

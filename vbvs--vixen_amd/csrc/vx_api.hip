@@ -100,25 +100,8 @@ __global__ void __launch_bounds__(kBlock) k_build_box(const uint32_t* __restrict
     box[i] = lx | ly << 3 | lz << 6 | hx << 9 | hy << 12 | hz << 15;
 }
 
-// Chunk of workgroup blockIdx.x when the XCDs take stripes of S consecutive
-// chunks in turn: XCD x runs workgroups i = 8k + x, and its k-th takes chunk
-// k % S of stripe 8 (k / S) + x; chunks past the last whole round of stripes go
-// to the XCDs in bands. Neighbouring rays still share an XCD's L2, while every
-// part of the image or ray set -- sky rows and ground rows, a call's camera
-// rays and its incoherent rays -- is spread over every XCD (contiguous bands
-// leave XCDs that drew cheap rays idle while the others finish).
-template <uint32_t S>
-__device__ __forceinline__ uint32_t xcd_stripe_chunk(uint32_t per_xcd)
-{
-    const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
-    const uint32_t full = per_xcd / S * S;
-    if (k < full) return ((k / S) * kXcds + x) * S + k % S;
-    return full * kXcds + x * (per_xcd - full) + (k - full);
-}
-
 // Stripe lengths: k_trace's chunks are 256 consecutive rays, k_render's 16x16 pixel tiles.
 constexpr uint32_t kTraceStripe = 4;
-constexpr uint32_t kRenderStripe = 16;
 
 // Every lane of the workgroup copies part of the descriptor prefix into LDS.
 template <bool STAGED>

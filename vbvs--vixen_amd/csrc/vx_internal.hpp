@@ -108,6 +108,25 @@ inline vx_status scene_usable(const vx_scene* s, const char* who)
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kMaxLdsBytes = 65536;
 
+// Chunk of workgroup blockIdx.x when the XCDs take stripes of S consecutive
+// chunks in turn: XCD x runs workgroups i = 8k + x, and its k-th takes chunk
+// k % S of stripe 8 (k / S) + x; chunks past the last whole round of stripes go
+// to the XCDs in bands. Neighbouring rays still share an XCD's L2, while every
+// part of the image or ray set -- sky rows and ground rows, a call's camera
+// rays and its incoherent rays -- is spread over every XCD (contiguous bands
+// leave XCDs that drew cheap rays idle while the others finish).
+template <uint32_t S>
+__device__ __forceinline__ uint32_t xcd_stripe_chunk(uint32_t per_xcd)
+{
+    const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+    const uint32_t full = per_xcd / S * S;
+    if (k < full) return ((k / S) * kXcds + x) * S + k % S;
+    return full * kXcds + x * (per_xcd - full) + (k - full);
+}
+
+// Stripe length of the image kernels (vx_render, the albedo passes): 16x16 pixel tiles.
+constexpr uint32_t kRenderStripe = 16;
+
 // Staged descriptor count for `depth` levels of `s` (0 = none).
 inline uint32_t staged_nodes(const vx_scene* s, uint32_t depth)
 {
