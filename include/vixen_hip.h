@@ -244,7 +244,9 @@ vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* b
  * shaders/VoxelRayMarch_Compressed.comp:283-305, brick DDA and hit semantics as
  * libraries/SVO/src/SVOBrickDDA.cpp:187-225,390-533, BC1 decode as
  * libraries/VoxelData/src/Compression/DXT1Compressor.cpp:14-36).
- * rays and hits are device pointers of n elements. */
+ * rays and hits are device pointers of n elements that must not overlap (a
+ * call's second launch reads rays after its first has written hit records).
+ * Until the call completes, hits may hold intermediate words. */
 vx_status vx_trace(vx_scene* scene, const vx_ray* rays, uint64_t n, vx_hit* hits, vx_stream stream);
 
 /* The compute pipeline's per-frame dispatch (VoxelRayMarch_Compressed.comp:493-594):

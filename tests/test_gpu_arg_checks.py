@@ -1,6 +1,6 @@
 """The Python mirror refuses buffers the kernels cannot use, before any launch:
 tensors on another device or not contiguous, hit buffers smaller than 32
-bytes per ray, batched ray offsets of the wrong length, decreasing or past
+bytes per ray or overlapping the rays, batched ray offsets of the wrong length, decreasing or past
 the rays (the C side reads offsets[n_scenes] and trusts the rest), and
 undersized pools of device-buffer scenes."""
 from __future__ import annotations
@@ -32,6 +32,12 @@ def test_undersized_and_foreign_buffers_are_refused(vx, cornell_bc1, gpu_ctx):
         scene.bounce_rays(rays, torch.empty(64, dtype=torch.uint8, device="cuda"), seed=1)
     with pytest.raises(ValueError, match="bytes"):
         vx.camera_rays(cam, 16, 16, 1e30, out=torch.empty(100, dtype=torch.uint8, device="cuda"))
+    with pytest.raises(ValueError, match="overlaps"):
+        scene.trace(rays, hits=rays)                              # vx_trace reads rays after writing hits
+    both = torch.empty(32 * 300, dtype=torch.uint8, device="cuda")
+    both[: 32 * 256] = rays
+    with pytest.raises(ValueError, match="overlaps"):
+        scene.trace_lod(both[: 32 * 256], vx.VxLod(0.0, 0.01), hits=both[32 * 40:])
     # a correct call still works after the refusals
     good = scene.trace(rays, hits=torch.empty(32 * 256, dtype=torch.uint8, device="cuda"))
     torch.cuda.synchronize()

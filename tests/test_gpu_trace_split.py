@@ -10,6 +10,10 @@ offsets that are not multiples of 64, so some blocks hold both kinds (they are
 incoherent blocks), ranges cut through blocks, and counts are not multiples of
 64. The large sizes reach past 128 rays per resident wave, so they take the
 two-launch form, and waves take several blocks and several refills.
+
+The first launch marks the parts of the blocks it leaves to the refill launch
+in the hit buffer itself (the first record of each part); the output buffer's
+earlier contents, marks included, must not matter.
 """
 from __future__ import annotations
 
@@ -90,3 +94,25 @@ def test_mixed_blocks_lod(vx, orc, cave, gpu_ctx, w, h, n_runs):
     assert ((got["flags"] >> 6) & 1).sum() > 0  # some LOD stops
     bad = record_diff(got, want)
     assert len(bad) == 0, f"{len(bad)} of {len(rays)} LOD records differ, first at {bad[0]}"
+
+
+def test_stale_output_contents_do_not_matter(vx, orc, cave, gpu_ctx):
+    """The output buffer starts filled with the refill mark (flags 6: face-axis field 3) in every record, then
+    with the records of another ray set; both calls must give the oracle's records."""
+    import torch
+    grid, cam, built = cave
+    rng = np.random.default_rng(31)
+    rays = _mixed_rays(orc, cam, 1440, 810, 256, rng, 2500)
+    other = _mixed_rays(orc, cam, 1440, 810, 256, np.random.default_rng(32), 700)[: len(rays)]
+    scene = vx.Scene(gpu_ctx, built)
+    want, _ = orc.trace(orc.OracleScene.from_built(built), rays, nthreads=8)
+    hits = torch.zeros(len(rays) * 8, dtype=torch.int32, device="cuda")
+    hits.view(-1, 8)[:, 0] = 6
+    for prefill in ("marks", "other records"):
+        if prefill == "other records":
+            scene.trace(vx.rays_to_device(other), hits.view(torch.uint8))
+        scene.trace(vx.rays_to_device(rays), hits.view(torch.uint8))
+        _sync()
+        got = vx.hits_to_numpy(hits.view(torch.uint8))
+        bad = record_diff(got, want)
+        assert len(bad) == 0, f"{prefill}: {len(bad)} of {len(rays)} records differ, first at {bad[0]}"

@@ -615,11 +615,15 @@ def _ray_offsets(offsets, n_scenes: int, n_rays: int) -> C.Array:
 
 
 def _hits_for(hits, n: int, like, device: int, what: str = "hits"):
-    """The caller's hit buffer (checked to hold n records) or a new one."""
+    """The caller's hit buffer (checked to hold n records and not to overlap the n input records of `like`)
+    or a new one."""
     torch = _torch()
     if hits is None:
         return torch.empty(n * 32, dtype=torch.uint8, device=like.device)
     _dev_buf(hits, what, device, 32 * n)
+    a, b = hits.data_ptr(), like.data_ptr()
+    if n and a < b + 32 * n and b < a + 32 * n:
+        raise ValueError(f"{what} overlaps the input rays")
     return hits
 
 

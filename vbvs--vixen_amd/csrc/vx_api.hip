@@ -133,10 +133,29 @@ __device__ __forceinline__ bool block_shares_origin(const float o[3])
     return __ballot(!same) == 0ull;
 }
 
+// Flags word that k_trace<..., SPLIT> writes into the hit record of the first
+// ray of each part of a block it leaves to k_trace_refill: face-axis field 3,
+// which no record of a traced ray holds (axis 0-2). k_trace_refill deals each
+// wave a range of positions in a scattered block order (below), so a block is
+// one part, or two when a range boundary falls inside it. A part's first ray is
+// traced only by the wave whose range holds the part, after that wave has read
+// the ray's record; every such record is written by the first launch, with this
+// mark or with the ray's own record, so the refill launch classifies a part by
+// one word instead of re-reading its origins.
+constexpr uint32_t kLeftToRefill = 3u << 1;
+
+// k_trace_refill's dealing, known to the first launch so it can mark the parts:
+// the range of positions per wave and the inverse of the block order's step
+// modulo the block count (block b is dealt at positions 64 k .. 64 k + 63,
+// k = b * inv_step mod n_blocks).
+struct RefillDeal {
+    uint64_t range, inv_step;
+};
+
 template <bool COUNT, bool STAGED, bool LOD = false, bool SPLIT = false>
 __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __restrict__ rays, uint64_t n,
                                                   uint32_t per_xcd, vx_hit* __restrict__ hits,
-                                                  unsigned long long* __restrict__ counters)
+                                                  unsigned long long* __restrict__ counters, RefillDeal deal)
 {
     stage_prefix<STAGED>(s);
     const uint64_t i = (uint64_t)xcd_stripe_chunk<kTraceStripe>(per_xcd) * kBlock + threadIdx.x;
@@ -148,7 +167,13 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
         const float o[3] = {a.x, a.y, a.z};
         const float tmin = a.w;
         const bool shared = block_shares_origin(o);
-        if (SPLIT && !shared) return; // wave-uniform: k_trace_refill traces this block
+        if (SPLIT && !shared) { // wave-uniform: k_trace_refill traces this block
+            const uint64_t k = ((i >> 6) * deal.inv_step) % ((n + 63u) >> 6); // below 2^32 each
+            const uint64_t p0 = k << 6, q = (p0 + deal.range - 1u) / deal.range * deal.range;
+            const uint64_t lane = i & 63u;
+            if (lane == 0u || (q < p0 + 64u && lane == q - p0)) hits[i].flags = kLeftToRefill;
+            return;
+        }
         const float4 b = rp[1];
         const float d[3] = {b.x, b.y, b.z};
         const float tmax = b.w;
@@ -179,9 +204,11 @@ __global__ void __launch_bounds__(kBlock) k_trace(SceneView s, const vx_ray* __r
 // bijection). Clusters of incoherent blocks -- a set of camera rays followed
 // by a set of random rays, say -- then spread evenly over the waves, with no
 // atomics or device state (the launch stays capturable and re-entrant), while
-// each wave still takes its rays 64 neighbours at a time. A wave classifies
-// each block it reaches exactly as k_trace classifies it and skips the blocks
-// that share an origin (k_trace<..., SPLIT> traced them). Leaf visits wait
+// each wave still takes its rays 64 neighbours at a time. A block may be
+// split between two waves' ranges; each wave reads the hit record of the first
+// ray of its part before it traces any ray of that part (no other wave traces
+// that ray) and skips the part unless k_trace<..., SPLIT> marked it
+// kLeftToRefill. Leaf visits wait
 // until 3/4 of the live lanes wait at one (trace_ray's POSTPONE). Each lane
 // executes exactly the reference's iteration sequence, so records equal
 // k_trace's bit for bit.
@@ -191,7 +218,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 
                    uint32_t refill_min, vx_hit* __restrict__ hits)
 {
     stage_prefix<STAGED>(s);
-    const uint32_t lane = threadIdx.x & 63u;
     const uint64_t v = (uint64_t)blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
     const uint64_t n_blocks = (n + 63u) >> 6;
     const uint64_t slots = n_blocks << 6; // dealt positions: every block counts 64, its rays past n are empty
@@ -217,16 +243,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 
                     }
                     const uint64_t stop = min((pos | 63u) + 1u, last); // the dealt block's part in the range
                     const uint64_t b = ((pos >> 6) * step) % n_blocks;
-                    const uint64_t ri = (b << 6) + lane;
-                    bool shared = true;
-                    if (ri < n) {
-                        const float4 a = reinterpret_cast<const float4*>(rays + ri)[0]; // origin and tmin
-                        const float o[3] = {a.x, a.y, a.z};
-                        shared = block_shares_origin(o);
-                    }
                     const uint64_t lo = (b << 6) + (pos & 63u), hi = min((b << 6) + ((stop - 1u) & 63u) + 1u, n);
                     pos = stop;
-                    if (__ballot(!shared) == 0ull || lo >= hi) continue; // every lane of the wave agrees
+                    if (lo >= hi || hits[lo].flags != kLeftToRefill) continue; // wave-uniform
                     cur = lo;
                     end = hi;
                 }
@@ -795,6 +814,20 @@ constexpr uint32_t kRefillMin = 16;
 // call stays one k_trace launch that postpones leaf visits in incoherent blocks.
 constexpr uint64_t kSplitMinRange = 128;
 
+// x with a * x = 1 mod m (a and m coprime, m < 2^32; 0 for m = 1).
+static uint64_t inverse_mod(uint64_t a, uint64_t m)
+{
+    int64_t r0 = (int64_t)m, r1 = (int64_t)(a % m), t0 = 0, t1 = 1;
+    while (r1 != 0) {
+        const int64_t q = r0 / r1;
+        std::swap(r0, r1);
+        r1 -= q * r0;
+        std::swap(t0, t1);
+        t1 -= q * t0;
+    }
+    return (uint64_t)(t0 < 0 ? t0 + (int64_t)m : t0) % m;
+}
+
 // vx_trace on one stream: one k_trace launch for short calls, else two launches,
 // k_trace<..., SPLIT> for the 64-ray blocks that share an origin and
 // k_trace_refill for the others. Records are the same bits either way.
@@ -822,19 +855,20 @@ static vx_status launch_trace_split(const vx_scene* s, const vx_lod* lod, const 
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)s->cu_count * (kBlock / 64);
     const uint64_t range = std::max<uint64_t>(64, (n_blocks * 64 + resident - 1) / resident);
     if (range < kSplitMinRange) {
-        if (staged) k_trace<false, true, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr);
-        else k_trace<false, false, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr);
+        if (staged) k_trace<false, true, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr, RefillDeal{0, 0});
+        else k_trace<false, false, LOD><<<grid, block, lds, st>>>(v, rays, n, per, hits, nullptr, RefillDeal{0, 0});
         VX_HIP(hipGetLastError());
         return VX_OK;
     }
-    if (vk.lds_count > 0) k_trace<false, true, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
-    else k_trace<false, false, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr);
-    VX_HIP(hipGetLastError());
-    const uint64_t waves = (n_blocks * 64 + range - 1) / range;
-    const dim3 rgrid(static_cast<unsigned>((waves + kBlock / 64 - 1) / (kBlock / 64)));
     // the block order's step: near n_blocks / golden ratio, odd and coprime to n_blocks
     uint64_t step = (uint64_t)((double)n_blocks * 0.6180339887498949) | 1u;
     while (std::gcd(step, n_blocks) != 1u) step += 2u;
+    const RefillDeal deal{range, inverse_mod(step % n_blocks, n_blocks)};
+    if (vk.lds_count > 0) k_trace<false, true, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr, deal);
+    else k_trace<false, false, LOD, true><<<grid, block, ldsk, st>>>(vk, rays, n, per, hits, nullptr, deal);
+    VX_HIP(hipGetLastError());
+    const uint64_t waves = (n_blocks * 64 + range - 1) / range;
+    const dim3 rgrid(static_cast<unsigned>((waves + kBlock / 64 - 1) / (kBlock / 64)));
     if (staged) k_trace_refill<true, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, step % n_blocks, kRefillMin, hits);
     else k_trace_refill<false, LOD><<<rgrid, block, lds, st>>>(v, rays, n, range, step % n_blocks, kRefillMin, hits);
     VX_HIP(hipGetLastError());
@@ -860,8 +894,8 @@ static vx_status launch_trace(vx_scene* s, const vx_ray* rays, uint64_t n, vx_hi
     } else if (counters) {
         VX_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(uint64_t), st));
         auto* c = reinterpret_cast<unsigned long long*>(counters);
-        if (staged) k_trace<true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
-        else k_trace<true, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c);
+        if (staged) k_trace<true, true><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c, RefillDeal{0, 0});
+        else k_trace<true, false><<<grid, block, lds, st>>>(v, rays, n, static_cast<uint32_t>(per), hits, c, RefillDeal{0, 0});
     } else {
         return launch_trace_split<false>(s, nullptr, rays, n, hits, grid, static_cast<uint32_t>(per), st);
     }
