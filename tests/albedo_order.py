@@ -4,10 +4,12 @@ float32 for the tests, and an adversarial layout of per-pixel rows.
 vx_albedo_backward (deterministic mode) sums each row's contributions
 dL/dpixel * shade in this order:
   * a row hit by at most 32 pixels: sequentially, in pixel order;
-  * a longer row: 256 threads, thread t adding the row's elements t, t+512, ...
-    (in sorted, that is pixel, order) into one partial sum and t+256, t+768, ...
-    into another, then s0 + s1 per thread and a halving tree over the threads
-    (h = 128, 64, ..., 1);
+  * a row hit by 33 to 4096 pixels: 64 lanes, lane t adding the row's elements
+    t, t+128, ... (in sorted, that is pixel, order) into one partial sum and
+    t+64, t+192, ... into another, then s0 + s1 per lane and a halving tree over
+    the lanes (h = 32, 16, ..., 1);
+  * a longer row: the same with 256 threads (elements t, t+512, ... and t+256,
+    t+768, ...; tree h = 128, 64, ..., 1);
 and adds each row's sum to the row's gradient. Misses carry row 0xFFFFFFFF.
 """
 from __future__ import annotations
@@ -16,7 +18,27 @@ import numpy as np
 
 MISS = 0xFFFFFFFF
 SHORT = 32
-LANES = 256
+WAVE_RUN = 4096  # the longest row summed by one wave of 64 lanes; longer rows use 256 threads
+
+
+def _tree_sum(v: np.ndarray, width: int) -> np.ndarray:
+    """Elements of v [n, 3] summed by `width` lanes: lane t adds t, t+2w, ... and t+w, t+3w, ..., then s0 + s1
+    and a halving tree over the lanes."""
+    n = len(v)
+    lane = np.arange(width)
+    s0 = np.zeros((width, 3), np.float32)
+    s1 = np.zeros((width, 3), np.float32)
+    for k0 in range(0, n, 2 * width):
+        i0, i1 = k0 + lane, k0 + width + lane
+        m0, m1 = i0 < n, i1 < n
+        s0[m0] = s0[m0] + v[i0[m0]]
+        s1[m1] = s1[m1] + v[i1[m1]]
+    part = s0 + s1
+    h = width // 2
+    while h > 0:
+        part[:h] = part[:h] + part[h:2 * h]
+        h //= 2
+    return part[0]
 
 
 def restated_gradient(rows: np.ndarray, shade: np.ndarray, gimg: np.ndarray, grad0: np.ndarray) -> np.ndarray:
@@ -38,23 +60,9 @@ def restated_gradient(rows: np.ndarray, shade: np.ndarray, gimg: np.ndarray, gra
         m = sl > k
         s[m] = s[m] + c[ps[sb[m] + k]]
     g[rs[sb]] = g[rs[sb]] + s
-    # long rows: the 256-thread order
-    lane = np.arange(LANES)
+    # long rows: the 64-lane order up to WAVE_RUN pixels, the 256-thread order beyond
     for b, n in zip(starts[~sh], lens[~sh]):
-        v = c[ps[b:b + n]]
-        s0 = np.zeros((LANES, 3), np.float32)
-        s1 = np.zeros((LANES, 3), np.float32)
-        for k0 in range(0, n, 2 * LANES):
-            i0, i1 = k0 + lane, k0 + LANES + lane
-            m0, m1 = i0 < n, i1 < n
-            s0[m0] = s0[m0] + v[i0[m0]]
-            s1[m1] = s1[m1] + v[i1[m1]]
-        part = s0 + s1
-        h = LANES // 2
-        while h > 0:
-            part[:h] = part[:h] + part[h:2 * h]
-            h //= 2
-        g[rs[b]] = g[rs[b]] + part[0]
+        g[rs[b]] = g[rs[b]] + _tree_sum(c[ps[b:b + n]], 64 if n <= WAVE_RUN else 256)
     return g
 
 
@@ -76,6 +84,8 @@ def adversarial_rows(rng, px: int, V: int) -> np.ndarray:
         put(int(r), 32)  # the longest short rows
     for r in picks[401:451]:
         put(int(r), int(rng.integers(500, 700)))  # long rows across windows
+    put(int(picks[451]), 4096)  # the longest row one wave sums
+    put(int(picks[452]), 4097)  # the shortest row the whole workgroup sums
     put(0, 7)
     put(V - 1, 40)
     # the rest: rows hit by 1-5 pixels, leaving about a tenth of the image as misses

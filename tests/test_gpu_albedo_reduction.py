@@ -4,14 +4,16 @@ vx_albedo_backward in deterministic mode sorts (row, pixel) pairs by row (stable
 pixel order within a row) and sums each row's contributions dL/dpixel * shade in
 a fixed order:
   * a row hit by at most 32 pixels: sequentially, in pixel order;
-  * a longer row: 256 threads, thread t adding the row's elements t, t+512, ...
-    into one partial sum and t+256, t+768, ... into another, then s0 + s1 per
-    thread and a halving tree over the threads (h = 128, 64, ..., 1);
+  * a row hit by 33 to 4096 pixels: 64 lanes, lane t adding the row's elements
+    t, t+128, ... into one partial sum and t+64, t+192, ... into another, then
+    s0 + s1 per lane and a halving tree over the lanes (h = 32, ..., 1);
+  * a longer row: the same with 256 threads (tree h = 128, ..., 1);
 and adds each row's sum to grad. With stored hits the per-pixel rows and shades
 are inputs, so adversarial layouts are fed directly: rows of exactly 32 and 33
-pixels (the short/long boundary), rows that cross the reduction's 512-position
-windows, one row covering 100 000 pixels (its end lies windows past its start),
-rows 0 and V-1, misses. The expected gradient is that order restated with
+pixels (the short/long boundary), of 4096 and 4097 (the wave/workgroup
+boundary), rows that cross the reduction's 512-position windows, one row
+covering 100 000 pixels (its end lies windows past its start), rows 0 and V-1,
+misses. The expected gradient is that order restated with
 numpy float32 arithmetic; the GPU result must equal it bit for bit, from a
 nonzero starting gradient (the backward accumulates). The cornell scene's row
 keys need two 8-bit sort passes, the cave's three.

@@ -17,9 +17,10 @@
 //   deterministic  (row, pixel) pairs are sorted by row with a stable LSD
 //                  radix sort written here (equal rows keep pixel order,
 //                  misses last); a run head sums its run in a fixed order (one
-//                  thread in sequence for runs of <= 32 pairs, else one
-//                  workgroup: thread t takes elements t, t+256, ..., then a
-//                  fixed tree), so the per-row sums, added without conflicts,
+//                  thread in sequence for runs of <= 32 pairs, one wave for
+//                  runs of <= 4096, else one workgroup: lane or thread t takes
+//                  elements t, t + width, ..., then a fixed tree), so the
+//                  per-row sums, added without conflicts,
 //                  depend only on the input: identical bits on every run. (A
 //                  decoupled look-back reduce-by-key is not used: the order in
 //                  which it combines the partial sums of a run spanning several
@@ -523,12 +524,75 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __res
     }
 }
 
+// Runs longer than this are summed by a whole workgroup, shorter long runs by one wave.
+constexpr uint32_t kWaveRun = 4096;
+
+// Sum of a run's contributions [b, e) by one wave in a fixed order: lane l adds
+// elements b+l, b+l+128, ... into one partial sum and b+l+64, b+l+192, ... into
+// another, loading four steps ahead; then s0 + s1 per lane and a halving tree
+// over the lanes (h = 32, 16, ..., 1: lane t < h adds lane t + h's value).
+// Every lane of the wave calls it; lane 0's result is the sum.
+__device__ __forceinline__ void wave_run_sum(const Rgb* __restrict__ vals_sorted, uint32_t b, uint32_t e,
+                                             uint32_t lane, float out[3])
+{
+    float s0[3] = {0.0f, 0.0f, 0.0f}, s1[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t i = b + lane;
+    for (; i + 7u * 64u < e; i += 8u * 64u) { // four steps: eight loads in flight
+        Rgb v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) v[k] = vals_sorted[i + k * 64u];
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; k += 2u) {
+            s0[0] += v[k].r; s0[1] += v[k].g; s0[2] += v[k].b;
+            s1[0] += v[k + 1].r; s1[1] += v[k + 1].g; s1[2] += v[k + 1].b;
+        }
+    }
+    for (; i + 64u < e; i += 128u) {
+        const Rgb v0 = vals_sorted[i], v1 = vals_sorted[i + 64u];
+        s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
+        s1[0] += v1.r; s1[1] += v1.g; s1[2] += v1.b;
+    }
+    if (i < e) {
+        const Rgb v0 = vals_sorted[i];
+        s0[0] += v0.r; s0[1] += v0.g; s0[2] += v0.b;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float x = s0[c] + s1[c];
+#pragma unroll
+        for (uint32_t h = 32u; h > 0u; h >>= 1) x += __shfl_down(x, h, 64); // lanes >= h: unused
+        out[c] = x;
+    }
+}
+
+// End of the run of `row` that starts at b and continues past its window: the
+// first position after b whose key differs, or n. Bisection by one wave with
+// one probe per lane (keys are sorted: the probes that still hold the row are a
+// prefix). Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_run_end(const uint32_t* __restrict__ keys_sorted, uint32_t n, uint32_t b,
+                                                 uint32_t row, uint32_t lane)
+{
+    // keys[lo - 1] == row; hi == n or keys[hi] != row; the end is in [lo, hi]
+    uint32_t lo = b + 1u, hi = n;
+    while (lo < hi) {
+        const uint32_t q = lo + (uint32_t)(((uint64_t)(hi - lo) * lane) / 64u);
+        const uint32_t held = (uint32_t)__popcll(__ballot(keys_sorted[q] == row)); // probes 0 .. held-1
+        const uint32_t nlo = held > 0u ? (uint32_t)__shfl((int)q, (int)held - 1, 64) + 1u : lo;
+        const uint32_t nhi = held < 64u ? (uint32_t)__shfl((int)q, (int)held, 64) : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
 // One workgroup per window of k_reduce_runs, taking the long runs [b, e) listed
-// in the window's slots in turn: an end listed as 0 lies past the window, and
-// the group finds it by bisection with one probe per thread (keys are sorted:
-// the probes that still hold the row are a prefix). Thread t then adds elements
-// t, t+256, t+512, ... in order into two interleaved partial sums, loading four
-// steps ahead, and the threads' sums are combined by a fixed tree in LDS.
+// in the window's slots (an end listed as 0 lies past the window and is found by
+// wave_run_end). Wave w takes slots w, w+4, ...: a run of at most kWaveRun pairs
+// is summed by that wave (wave_run_sum); a longer one is set aside and then
+// summed by the whole workgroup: thread t adds elements t, t+256, t+512, ... in
+// order into two interleaved partial sums (t, t+512, ... and t+256, t+768, ...),
+// loading four steps ahead, and the threads' sums are combined by a fixed tree in
+// LDS. Runs of different rows: no conflicts.
 __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* __restrict__ keys_sorted,
                                                                const Rgb* __restrict__ vals_sorted, uint32_t n,
                                                                const uint2* __restrict__ long_runs,
@@ -536,29 +600,33 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* 
                                                                float* __restrict__ grad)
 {
     __shared__ float part[3][kAlbBlock];
-    __shared__ uint32_t probe[kAlbBlock];
-    const uint32_t t = threadIdx.x;
+    __shared__ uint2 big[kLongPerWindow]; // runs longer than kWaveRun, ends resolved
+    __shared__ uint32_t n_big;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t count = long_count[blockIdx.x];
-    for (uint32_t j = 0; j < count; ++j) {
+    if (t == 0u) n_big = 0u;
+    __syncthreads();
+    for (uint32_t j = wave; j < count; j += kAlbBlock / 64u) {
         const uint2 run = long_runs[(size_t)blockIdx.x * kLongPerWindow + j];
-        const uint32_t b = run.x;
-        const uint32_t row = keys_sorted[b];
-        uint32_t e = run.y;
-        if (e == 0u) {
-            // keys[lo - 1] == row; hi == n or keys[hi] != row; the end is in [lo, hi]
-            uint32_t lo = b + 1u, hi = n;
-            while (lo < hi) {
-                const uint32_t q = lo + (uint32_t)(((uint64_t)(hi - lo) * t) / kAlbBlock);
-                probe[t] = q;
-                const int held = __syncthreads_count(keys_sorted[q] == row); // probes 0 .. held-1 hold the row
-                const uint32_t nlo = held > 0 ? probe[held - 1] + 1u : lo;
-                const uint32_t nhi = held < (int)kAlbBlock ? probe[held] : hi;
-                __syncthreads(); // probe[] is rewritten by the next round
-                lo = nlo;
-                hi = nhi;
-            }
-            e = lo;
+        const uint32_t b = run.x, row = keys_sorted[b];
+        const uint32_t e = run.y != 0u ? run.y : wave_run_end(keys_sorted, n, b, row, lane);
+        if (e - b > kWaveRun) {
+            if (lane == 0u) big[atomicAdd(&n_big, 1u)] = make_uint2(b, e); // summed below, in slot order
+            continue;
         }
+        float sum[3];
+        wave_run_sum(vals_sorted, b, e, lane, sum);
+        if (lane == 0u) {
+            grad[(size_t)row * 3 + 0] += sum[0];
+            grad[(size_t)row * 3 + 1] += sum[1];
+            grad[(size_t)row * 3 + 2] += sum[2];
+        }
+    }
+    __syncthreads();
+    const uint32_t nb = n_big; // each big run's sum depends on the run alone, not on its place in big[]
+    for (uint32_t j = 0; j < nb; ++j) {
+        const uint32_t b = big[j].x, e = big[j].y;
+        const uint32_t row = keys_sorted[b];
         float s0[3] = {0.0f, 0.0f, 0.0f}, s1[3] = {0.0f, 0.0f, 0.0f};
         uint32_t i = b + t;
         for (; i + 7u * kAlbBlock < e; i += 8u * kAlbBlock) { // four steps: eight loads in flight
@@ -597,7 +665,7 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* 
             grad[(size_t)row * 3 + 1] += part[1][0];
             grad[(size_t)row * 3 + 2] += part[2][0];
         }
-        __syncthreads(); // part[] and probe[] are reused by the next run
+        __syncthreads(); // part[] is reused by the next run
     }
 }
 
