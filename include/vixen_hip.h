@@ -390,9 +390,11 @@ vx_status vx_albedo_forward(vx_scene* scene, const vx_camera* camera, uint32_t w
 
 /* Adjoint: grad_albedo[row] += grad_image[p] * shade[p] over the pixels p that
  * hit the row. hit_row/hit_shade from vx_albedo_forward, or NULL to re-trace the
- * hits (a stored row at or above the scene's voxel count counts as a miss). mode VX_GRAD_ATOMIC (FP32 atomics) or VX_GRAD_DETERMINISTIC (stable sort
- * by row, then each row's run summed in a fixed order: identical bits on every
- * run; needs width*height < 2^31). grad_albedo accumulates. The scene holds the
+ * hits (a stored row at or above the scene's voxel count counts as a miss).
+ * mode VX_GRAD_ATOMIC (FP32 atomics) or VX_GRAD_DETERMINISTIC (stable sort by
+ * row, then each row's run summed in a fixed order that depends only on the
+ * run's length: identical bits on every run; needs width*height < 2^31).
+ * grad_albedo accumulates. The scene holds the
  * albedo calls' row table and sort workspace (built and grown on first use), so
  * albedo calls on one scene must not overlap; tracing and rendering only read it. */
 vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t width, uint32_t height,
