@@ -20,10 +20,11 @@ More legs ride in the same JSON line:
             random rays from inside the volume, and one seeded cosine bounce
             ray per primary hit of the headline frame.
   rows      the rows either side of the path on the same scene: the VSCA v1
-            scene cache written and read back (host), the compressed
-            shader's RGBA8 image (with the DXT normal pool), and screen-space
-            LOD on the frame's rays at two cone widths; each output compared
-            with the oracle's unless --no-cpu.
+            scene cache written and read back (host), the scene's BC1 pool
+            re-encoded on the device (compared with the host builder's), the
+            compressed shader's RGBA8 image (with the DXT normal pool), and
+            screen-space LOD on the frame's rays at two cone widths; each
+            output compared with the oracle's unless --no-cpu.
   urban     the 1024^3 urban box grid (BC1, far pointers, 7 levels above the
             bricks) at 2560x1440: vx_render of the primary rays, and vx_trace of
             one seeded cosine bounce ray per primary hit; both ray sets compared
@@ -403,6 +404,19 @@ def rows_leg(vx, orc, torch, ctx, stream, built, cam, W, H, steps: int, check: b
     out["ingest"] = {"workload": "cave256 scene as a VSCA v1 cache file (host)", "file_mb": round(size / 1e6, 2),
                      "write_ms": round(t_w * 1e3, 2), "read_ms": round(t_r * 1e3, 2),
                      "read_gbps": round(size / t_r / 1e9, 2), "byte_identical": bool(same)}
+
+    # the scene's BC1 pool re-encoded on the device from its brick pool (vx_bc1_encode_device)
+    bricks = torch.from_numpy(built.brick_data.view(np.int32)).cuda()
+    pool = torch.empty((len(built.brick_data), 32), dtype=torch.int64, device="cuda")
+    ms_enc = timed(torch, stream, lambda: vx.bc1_encode_device(bricks, out=pool, stream=stream), steps, 2)
+    torch.cuda.synchronize()
+    n_blocks = pool.numel()
+    out["bc1_encode"] = {"workload": f"cave256 brick pool, {len(built.brick_data)} bricks, {n_blocks} BC1 blocks "
+                                     "of 16 voxels (device)",
+                         "ms": round(ms_enc, 4), "mblocks_per_s": round(n_blocks / ms_enc / 1e3, 1),
+                         "bit_identical_to_host_builder": bool(pool.cpu().numpy().view(np.uint64).tobytes() ==
+                                                               built.bc1.tobytes())}
+    del bricks, pool
 
     t0 = time.perf_counter()
     built.encode_normals()

@@ -231,6 +231,23 @@ vx_status vx_scene_retain(vx_scene* scene);
  * they made at vx_batch_create. */
 vx_status vx_scene_revalidate(vx_scene* scene);
 
+/* BC1 pool of a brick pool on the current device: bc1_blocks[32 b + k] encodes
+ * voxels 16k .. 16k+15 (linear x + 8y + 64z order) of brick b, exactly as the
+ * host builder's pool (DXT1ColorCompressor::encodeBlockTyped with validCount
+ * 16, libraries/VoxelData/src/Compression/DXT1Compressor.cpp:38-111, as
+ * LaineKarrasOctree::rebuild applies it, libraries/SVO/src/SVORebuild.cpp:680-715).
+ * brick_data (brick_count x 512 words, 16-byte aligned) and bc1_blocks
+ * (brick_count x 32 u64, 8-byte aligned) are device pointers. A voxel word is an
+ * RGBA8 colour when palette is NULL, else a material id coloured by the device
+ * array palette of 512 RGB floats: entry id for ids below 256, entry
+ * 256 + id mod 256 for larger ids (vx_material_id_to_color of ids 0-511 gives
+ * the builder's VX_BC1_FROM_MATERIAL pool; 256 colours repeated twice give a
+ * colour per id byte). Word 0 is empty and an all-empty block is 0. No reference counterpart on the device (the reference encodes on
+ * the CPU): a caller that edits a vx_scene_create_device scene's colours in
+ * place re-encodes its BC1 pool with it, then calls vx_scene_revalidate. */
+vx_status vx_bc1_encode_device(const uint32_t* brick_data, uint64_t brick_count, const float* palette,
+                               uint64_t* bc1_blocks, vx_stream stream);
+
 /* Descriptor/brick counts and tree shape of an uploaded scene. */
 vx_status vx_scene_info(const vx_scene* scene, uint64_t* node_count, uint64_t* brick_count,
                         int32_t* user_max_levels, int32_t* bricks_per_axis);

@@ -176,6 +176,7 @@ def _load() -> C.CDLL:
         "vx_build_from_bricks": ([vp, vp, u64, u32, C.c_int, C.c_int, C.POINTER(C.POINTER(VxBuiltScene))], st),
         "vx_built_scene_free": ([C.POINTER(VxBuiltScene)], None),
         "vx_bc1_encode_block": ([vp], u64),
+        "vx_bc1_encode_device": ([vp, u64, vp, vp, vp], st),
         "vx_encode_normals": ([vp, u64, C.c_int, vp], st),
         "vx_cache_read": ([C.c_char_p, C.POINTER(vp)], st),
         "vx_cache_entry_count": ([vp, C.POINTER(u32)], st),
@@ -221,7 +222,7 @@ EXPORTED = ("vx_last_error", "vx_version", "vx_ctx_create", "vx_ctx_destroy", "v
             "vx_cache_entry_count", "vx_cache_entry", "vx_cache_free", "vx_cache_write", "vx_trace_lod",
             "vx_lod_from_camera", "vx_lod_from_camera_near", "vx_lod_with_bias", "vx_scene_create_device",
             "vx_albedo_backward_noalloc", "vx_material_color", "vx_material_palette", "vx_material_id_to_color",
-            "vx_scene_revalidate")
+            "vx_scene_revalidate", "vx_bc1_encode_device")
 
 
 def lod_from_camera(fov_y: float, screen_height: int, near_plane: float | None = None) -> VxLod:
@@ -405,6 +406,34 @@ def _take_built(out, bc1: bool) -> BuiltScene:
 def bc1_encode_block(colors: np.ndarray) -> int:
     c = np.ascontiguousarray(colors, dtype=np.float32).reshape(16, 3)
     return int(LIB.vx_bc1_encode_block(_ptr(c)))
+
+
+def material_id_palette() -> np.ndarray:
+    """float32 [512, 3]: MaterialIdToColor of ids 0-511, the palette with which bc1_encode_device reproduces the
+    builder's material-id BC1 pool (entries 256-511 colour the ids of 256 and above by id mod 256)."""
+    return np.stack([material_id_to_color(i) for i in range(512)]).astype(np.float32)
+
+
+def bc1_encode_device(brick_data, palette=None, out=None, stream=None):
+    """The BC1 pool of a device brick pool (torch int32/uint32 tensor of brick_count x 512 words, RGBA8 colours, or
+    material ids when `palette` (a device float32 [512, 3] tensor, see material_id_palette) is given): an int64
+    tensor [brick_count, 32], bit-identical to the host builder's. vx_bc1_encode_device."""
+    torch = _torch()
+    if brick_data.dtype not in (torch.int32, torch.uint32) or brick_data.numel() % 512:
+        raise ValueError("brick_data must be 32-bit words, 512 per brick")
+    nb = brick_data.numel() // 512
+    device = brick_data.device.index if brick_data.is_cuda else -1
+    _dev_buf(brick_data, "brick_data", device)
+    if palette is not None:
+        _dev_buf(palette, "palette", device, 512 * 3 * 4)
+        if palette.dtype != torch.float32:
+            raise ValueError("palette must be float32 [512, 3]")
+    if out is None:
+        out = torch.empty((nb, 32), dtype=torch.int64, device=brick_data.device)
+    _dev_buf(out, "out", device, nb * 32 * 8)
+    _check(LIB.vx_bc1_encode_device(brick_data.data_ptr(), nb, palette.data_ptr() if palette is not None else None,
+                                    out.data_ptr(), _stream_ptr(stream)), "vx_bc1_encode_device")
+    return out
 
 
 def dxt_normal_encode_block(normals: np.ndarray) -> tuple[int, int]:

@@ -42,7 +42,7 @@ HOST_SOURCES = ["vx_builder.cpp", "vx_scenes.cpp", "vx_normals.cpp", "vx_cache.c
 # host code that calls the HIP runtime API (no kernels): the C++ ISVOStructure adapter
 HIP_HOST_SOURCES = ["vx_svo.cpp"]
 HIP_HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include"]
-HIP_SOURCES = ["vx_api.hip", "vx_batch.hip", "vx_albedo.hip"]
+HIP_SOURCES = ["vx_api.hip", "vx_batch.hip", "vx_albedo.hip", "vx_encode.hip"]
 
 
 def _run(cmd: list[str]) -> None:
