@@ -9,6 +9,8 @@ Python mirror of the reference's ray-cast surface over the C ABI in
   ``VoxelSceneCacher::BuildOctree``, libraries/CashSystem/src/VoxelSceneCacher.cpp:463-628);
   ``read_cache`` / ``write_cache`` -- VIXEN's VSCA v1 scene cache
   (libraries/CashSystem/src/VoxelSceneCacher.cpp:179-364);
+  ``bc1_encode_device`` -- the rebuild's BC1 pass on a device brick pool,
+  bit-identical to the host builder's pool;
 * ``Scene.trace`` -- the batch form of ``ISVOStructure::castRay``
   (libraries/SVO/include/ISVOStructure.h:122-126);
 * ``Scene.render`` -- the compute pipeline's per-frame dispatch
