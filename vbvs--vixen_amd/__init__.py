@@ -433,8 +433,10 @@ def bc1_encode_device(brick_data, palette=None, out=None, stream=None):
     if out is None:
         out = torch.empty((nb, 32), dtype=torch.int64, device=brick_data.device)
     _dev_buf(out, "out", device, nb * 32 * 8)
-    _check(LIB.vx_bc1_encode_device(brick_data.data_ptr(), nb, palette.data_ptr() if palette is not None else None,
-                                    out.data_ptr(), _stream_ptr(stream)), "vx_bc1_encode_device")
+    with torch.cuda.device(device):  # the C call runs on the current device
+        _check(LIB.vx_bc1_encode_device(brick_data.data_ptr(), nb,
+                                        palette.data_ptr() if palette is not None else None, out.data_ptr(),
+                                        _stream_ptr(stream)), "vx_bc1_encode_device")
     return out
 
 
