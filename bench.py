@@ -22,9 +22,10 @@ More legs ride in the same JSON line:
   rows      the rows either side of the path on the same scene: the VSCA v1
             scene cache written and read back (host), the scene's BC1 pool
             re-encoded on the device (compared with the host builder's), the
-            compressed shader's RGBA8 image (with the DXT normal pool), and
-            screen-space LOD on the frame's rays at two cone widths; each
-            output compared with the oracle's unless --no-cpu.
+            compressed shader's RGBA8 image (with the DXT normal pool),
+            screen-space LOD on the frame's rays at two cone widths (each
+            output compared with the oracle's unless --no-cpu), and the same
+            rays handed over in host memory (PCIe-inclusive rate).
   urban     the 1024^3 urban box grid (BC1, far pointers, 7 levels above the
             bricks) at 2560x1440: vx_render of the primary rays, and vx_trace of
             one seeded cosine bounce ray per primary hit; both ray sets compared
@@ -460,6 +461,25 @@ def rows_leg(vx, orc, torch, ctx, stream, built, cam, W, H, steps: int, check: b
             row["parity"] = "bit-exact vs oracle" if got.tobytes() == want.tobytes() else "differs from the oracle"
         lod[f"bias_{int(bias)}"] = row
     out["lod"] = lod
+
+    # the boundary with host buffers: the frame's rays handed over in host memory and the records returned to it
+    # (LaineKarrasOctreeHip.castRays: copy in, vx_trace, copy out, synchronised), PCIe transfers included
+    adapter = vx.LaineKarrasOctreeHip(scene)
+    host_in = rays.cpu().numpy().view(vx.RAY_DTYPE)
+    adapter.castRays(host_in)  # warm-up
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        host_out = adapter.castRays(host_in)
+        ts.append(time.perf_counter() - t0)
+    ms_host = float(np.median(ts)) * 1e3
+    vx._check(vx.LIB.vx_trace(scene._h, rays.data_ptr(), W * H, hits.data_ptr(), stream.cuda_stream), "vx_trace")
+    torch.cuda.synchronize()
+    out["host_buffers"] = {"workload": "the 1080p frame's 2073600 rays in host memory, hit records back to host "
+                                       "memory (castRays: 66 MB each way over PCIe), median of 5",
+                           "ms": round(ms_host, 3), "mrays": round(W * H / ms_host / 1e3, 1),
+                           "device_resident_ms": round(ms_plain, 4),
+                           "same_records": bool(host_out.tobytes() == hits.cpu().numpy().tobytes())}
     scene.close()
     return out
 
