@@ -1,8 +1,8 @@
 """ctypes front of the CPU oracle (liboracle_vx.so). TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
-this module, and only as the checker (or the timed CPU baseline), never as the
-product path. See vx_oracle.c for what each function restates and how it is
+this module, and only to check results (or as the timed CPU baseline), never
+as the product path. See vx_oracle.c for what each function restates and how it is
 pinned.
 """
 from __future__ import annotations

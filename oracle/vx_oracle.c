@@ -1,6 +1,6 @@
 /*
  * vx_oracle.c -- CPU oracle of the voxel ray-cast hot path. TEST INFRASTRUCTURE:
- * the checker for the HIP kernels, never linked into the product.
+ * what the HIP kernels' results are checked against, never linked into the product.
  *
  * Built with strict IEEE FP32 (-O3 -ffp-contract=off, no fast-math). Every
  * expression keeps the evaluation order written here; the HIP kernel must match

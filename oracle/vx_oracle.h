@@ -1,7 +1,7 @@
 /*
  * vx_oracle.h -- CPU oracle for the voxel ray-cast hot path (TEST INFRASTRUCTURE).
  *
- * This is the checker, never the product: only tests/, __graft_entry__.smoke()
+ * It checks results and is never the product: only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg load it. It restates, in strict IEEE FP32 C
  * (built with -ffp-contract=off, no fast-math), the reference's traversal:
  *   - ray set-up and volume entry   libraries/SVO/src/SVOTraversal.cpp:117-149,238-259,516-583
