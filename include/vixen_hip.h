@@ -423,8 +423,8 @@ vx_status vx_albedo_backward(vx_scene* scene, const vx_camera* camera, uint32_t 
  * serving loops, harnesses that forbid hipMalloc). Same arguments, results
  * and modes; bit-identical gradients. It calls no hipMalloc/hipFree: the
  * deterministic mode's workspace (about 41 bytes per pixel: keys, pixel indices
- * and contributions with their sorted copies, the sort's per-tile digit counts
- * and the long-run slots) lives in a fixed 48 MiB device array that the library holds on
+ * and contributions with their sorted copies, and the sort's per-tile digit
+ * counts) lives in a fixed 48 MiB device array that the library holds on
  * every device it runs on until the process ends, so width * height is at most
  * VX_NOALLOC_MAX_PIXELS (larger images: VX_ERR_INVALID_ARG). The scene's voxel
  * row table must exist already: vx_scene_voxel_count or a vx_albedo_forward on

@@ -412,12 +412,11 @@ __global__ void __launch_bounds__(kSortThreads)
 // a run still open at the window's end is either closed within the next
 // kShortRun positions (independent loads) or certainly long. The runs of at
 // most kShortRun pairs are compacted into an LDS list and added in order, one
-// thread per run. Longer runs (voxels that cover many pixels) go to the
-// window's kLongPerWindow slots of the long-run list, with their end, or with
-// end 0 when it lies past the window, for k_reduce_long_runs (no global
-// counter: a single atomic counter shared by every window serialised badly on
-// views with many long runs). Misses (key miss_key) sort last and are skipped.
-// Rows are unique across runs: no conflicts.
+// thread per run. Longer runs (voxels that cover many pixels) go to a second
+// LDS list of at most kLongPerWindow runs, with their end, or with end 0 when it
+// lies past the window, and the same workgroup then sums them (sum_long_runs).
+// Misses (key miss_key) sort last and are skipped. Rows are unique across runs:
+// no conflicts.
 constexpr uint32_t kShortRun = 32;
 constexpr uint32_t kRunItems = 2;
 constexpr uint32_t kRunWindow = kAlbBlock * kRunItems;
@@ -425,104 +424,6 @@ static_assert(kAlbBlock == kSortThreads, "k_reduce_runs uses block_exclusive_sca
 static_assert(32u % kRunItems == 0u && kRunWindow / 32u <= kAlbBlock, "a thread's run starts fit in one mask word");
 // long runs start at least kShortRun + 1 positions apart
 constexpr uint32_t kLongPerWindow = kRunWindow / (kShortRun + 1u) + 1u;
-
-__global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __restrict__ keys_sorted,
-                                                          const Rgb* __restrict__ vals_sorted, uint32_t n,
-                                                          uint32_t miss_key, float* __restrict__ grad,
-                                                          uint2* __restrict__ long_runs, uint32_t* __restrict__ long_count)
-{
-    __shared__ uint32_t starts[kRunWindow / 32]; // bit i: window position i starts a run
-    __shared__ uint2 runs[kRunWindow];           // short runs: first position, end
-    __shared__ uint32_t run_row[kRunWindow];
-    const uint32_t t = threadIdx.x;
-    const uint32_t w0 = blockIdx.x * kRunWindow, p0 = w0 + t * kRunItems;
-    uint32_t row[kRunItems];
-#pragma unroll
-    for (uint32_t j = 0; j < kRunItems; ++j) row[j] = p0 + j < n ? keys_sorted[p0 + j] : 0u;
-    const uint32_t before = (p0 > 0u && p0 - 1u < n) ? keys_sorted[p0 - 1u] : ~row[0];
-    uint32_t mine = 0; // this thread's run starts, bit j = position p0 + j
-#pragma unroll
-    for (uint32_t j = 0; j < kRunItems; ++j) {
-        const uint32_t prev = j == 0u ? before : row[j - 1u];
-        if (p0 + j < n && prev != row[j]) mine |= 1u << j;
-    }
-    if (t < kRunWindow / 32u) starts[t] = 0u;
-    __syncthreads();
-    atomicOr(&starts[(t * kRunItems) >> 5], mine << ((t * kRunItems) & 31u)); // OR: order-free
-    __syncthreads();
-    uint32_t end[kRunItems];
-    uint32_t n_short = 0, n_longs = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kRunItems; ++j) {
-        end[j] = 0u;
-        if (!((mine >> j) & 1u)) continue;
-        // the next run start after window position L, else past the window
-        const uint32_t L = t * kRunItems + j;
-        uint32_t wi = L >> 5, bits = (L & 31u) == 31u ? 0u : starts[wi] & (~0u << ((L & 31u) + 1u));
-        while (bits == 0u && ++wi < kRunWindow / 32u) bits = starts[wi];
-        if (row[j] >= miss_key) continue; // the misses' run
-        const uint32_t p = p0 + j, wend = w0 + kRunWindow;
-        uint32_t e = 0u; // 0: past the window and longer than kShortRun
-        if (bits != 0u) {
-            e = w0 + wi * 32u + (uint32_t)(__ffs(bits) - 1);
-        } else if (wend >= n) {
-            e = n;
-        } else if (wend - p <= kShortRun) {
-            // positions p .. wend - 1 are in the run: is it closed within kShortRun + 1 pairs?
-            uint32_t differ = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < kShortRun; ++k) {
-                const uint32_t q = wend + k;
-                if (q < p + kShortRun + 1u && (q >= n || keys_sorted[q < n ? q : 0u] != row[j])) differ |= 1u << k;
-            }
-            if (differ != 0u) e = wend + (uint32_t)(__ffs(differ) - 1);
-            e = e < n ? e : n;
-        }
-        end[j] = e;
-        if (e == 0u || e - p > kShortRun) ++n_longs;
-        else ++n_short;
-    }
-    uint32_t total, total_long;
-    uint32_t slot = block_exclusive_scan(n_short, total);
-    uint32_t lslot = block_exclusive_scan(n_longs, total_long);
-    uint2* my_long = long_runs + (size_t)blockIdx.x * kLongPerWindow;
-#pragma unroll
-    for (uint32_t j = 0; j < kRunItems; ++j) {
-        const uint32_t p = p0 + j;
-        if (!((mine >> j) & 1u) || row[j] >= miss_key) continue;
-        if (end[j] != 0u && end[j] - p <= kShortRun) {
-            runs[slot] = make_uint2(p, end[j]);
-            run_row[slot] = row[j];
-            ++slot;
-        } else {
-            my_long[lslot++] = make_uint2(p, end[j]);
-        }
-    }
-    if (threadIdx.x == 0u) long_count[blockIdx.x] = total_long;
-    __syncthreads();
-    for (uint32_t r = t; r < total; r += kAlbBlock) {
-        const uint2 run = runs[r];
-        // added in order, four loads in flight at a time
-        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-        for (uint32_t i = run.x; i < run.y; i += 4u) {
-            Rgb v[4];
-#pragma unroll
-            for (uint32_t k = 0; k < 4u; ++k)
-                if (i + k < run.y) v[k] = vals_sorted[i + k];
-#pragma unroll
-            for (uint32_t k = 0; k < 4u; ++k)
-                if (i + k < run.y) {
-                    sr += v[k].r;
-                    sg += v[k].g;
-                    sb += v[k].b;
-                }
-        }
-        const size_t g = (size_t)run_row[r] * 3;
-        grad[g + 0] += sr;
-        grad[g + 1] += sg;
-        grad[g + 2] += sb;
-    }
-}
 
 // Runs longer than this are summed by a whole workgroup, shorter long runs by one wave.
 constexpr uint32_t kWaveRun = 4096;
@@ -585,29 +486,26 @@ __device__ __forceinline__ uint32_t wave_run_end(const uint32_t* __restrict__ ke
     return lo;
 }
 
-// One workgroup per window of k_reduce_runs, taking the long runs [b, e) listed
-// in the window's slots (an end listed as 0 lies past the window and is found by
-// wave_run_end). Wave w takes slots w, w+4, ...: a run of at most kWaveRun pairs
-// is summed by that wave (wave_run_sum); a longer one is set aside and then
-// summed by the whole workgroup: thread t adds elements t, t+256, t+512, ... in
-// order into two interleaved partial sums (t, t+512, ... and t+256, t+768, ...),
-// loading four steps ahead, and the threads' sums are combined by a fixed tree in
-// LDS. Runs of different rows: no conflicts.
-__global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* __restrict__ keys_sorted,
-                                                               const Rgb* __restrict__ vals_sorted, uint32_t n,
-                                                               const uint2* __restrict__ long_runs,
-                                                               const uint32_t* __restrict__ long_count,
-                                                               float* __restrict__ grad)
+// A window's long runs [b, e), listed in LDS by k_reduce_runs (an end listed as
+// 0 lies past the window and is found by wave_run_end). Wave w takes runs w,
+// w+4, ...: a run of at most kWaveRun pairs is summed by that wave
+// (wave_run_sum); a longer one is set aside and then summed by the whole
+// workgroup: thread t adds elements t, t+256, t+512, ... in order into two
+// interleaved partial sums (t, t+512, ... and t+256, t+768, ...), loading four
+// steps ahead, and the threads' sums are combined by a fixed tree in LDS. Runs
+// of different rows: no conflicts. Every thread of the workgroup calls it.
+__device__ __forceinline__ void sum_long_runs(const uint32_t* __restrict__ keys_sorted,
+                                              const Rgb* __restrict__ vals_sorted, uint32_t n, const uint2* lruns,
+                                              uint32_t count, float* __restrict__ grad)
 {
     __shared__ float part[3][kAlbBlock];
     __shared__ uint2 big[kLongPerWindow]; // runs longer than kWaveRun, ends resolved
     __shared__ uint32_t n_big;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const uint32_t count = long_count[blockIdx.x];
     if (t == 0u) n_big = 0u;
     __syncthreads();
     for (uint32_t j = wave; j < count; j += kAlbBlock / 64u) {
-        const uint2 run = long_runs[(size_t)blockIdx.x * kLongPerWindow + j];
+        const uint2 run = lruns[j];
         const uint32_t b = run.x, row = keys_sorted[b];
         const uint32_t e = run.y != 0u ? run.y : wave_run_end(keys_sorted, n, b, row, lane);
         if (e - b > kWaveRun) {
@@ -669,14 +567,110 @@ __global__ void __launch_bounds__(kAlbBlock) k_reduce_long_runs(const uint32_t* 
     }
 }
 
+
+__global__ void __launch_bounds__(kAlbBlock) k_reduce_runs(const uint32_t* __restrict__ keys_sorted,
+                                                          const Rgb* __restrict__ vals_sorted, uint32_t n,
+                                                          uint32_t miss_key, float* __restrict__ grad)
+{
+    __shared__ uint32_t starts[kRunWindow / 32]; // bit i: window position i starts a run
+    __shared__ uint2 runs[kRunWindow];           // short runs: first position, end
+    __shared__ uint32_t run_row[kRunWindow];
+    __shared__ uint2 lruns[kLongPerWindow];      // long runs: first position, end (0: past the window)
+    const uint32_t t = threadIdx.x;
+    const uint32_t w0 = blockIdx.x * kRunWindow, p0 = w0 + t * kRunItems;
+    uint32_t row[kRunItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) row[j] = p0 + j < n ? keys_sorted[p0 + j] : 0u;
+    const uint32_t before = (p0 > 0u && p0 - 1u < n) ? keys_sorted[p0 - 1u] : ~row[0];
+    uint32_t mine = 0; // this thread's run starts, bit j = position p0 + j
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        const uint32_t prev = j == 0u ? before : row[j - 1u];
+        if (p0 + j < n && prev != row[j]) mine |= 1u << j;
+    }
+    if (t < kRunWindow / 32u) starts[t] = 0u;
+    __syncthreads();
+    atomicOr(&starts[(t * kRunItems) >> 5], mine << ((t * kRunItems) & 31u)); // OR: order-free
+    __syncthreads();
+    uint32_t end[kRunItems];
+    uint32_t n_short = 0, n_longs = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        end[j] = 0u;
+        if (!((mine >> j) & 1u)) continue;
+        // the next run start after window position L, else past the window
+        const uint32_t L = t * kRunItems + j;
+        uint32_t wi = L >> 5, bits = (L & 31u) == 31u ? 0u : starts[wi] & (~0u << ((L & 31u) + 1u));
+        while (bits == 0u && ++wi < kRunWindow / 32u) bits = starts[wi];
+        if (row[j] >= miss_key) continue; // the misses' run
+        const uint32_t p = p0 + j, wend = w0 + kRunWindow;
+        uint32_t e = 0u; // 0: past the window and longer than kShortRun
+        if (bits != 0u) {
+            e = w0 + wi * 32u + (uint32_t)(__ffs(bits) - 1);
+        } else if (wend >= n) {
+            e = n;
+        } else if (wend - p <= kShortRun) {
+            // positions p .. wend - 1 are in the run: is it closed within kShortRun + 1 pairs?
+            uint32_t differ = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kShortRun; ++k) {
+                const uint32_t q = wend + k;
+                if (q < p + kShortRun + 1u && (q >= n || keys_sorted[q < n ? q : 0u] != row[j])) differ |= 1u << k;
+            }
+            if (differ != 0u) e = wend + (uint32_t)(__ffs(differ) - 1);
+            e = e < n ? e : n;
+        }
+        end[j] = e;
+        if (e == 0u || e - p > kShortRun) ++n_longs;
+        else ++n_short;
+    }
+    uint32_t total, total_long;
+    uint32_t slot = block_exclusive_scan(n_short, total);
+    uint32_t lslot = block_exclusive_scan(n_longs, total_long);
+#pragma unroll
+    for (uint32_t j = 0; j < kRunItems; ++j) {
+        const uint32_t p = p0 + j;
+        if (!((mine >> j) & 1u) || row[j] >= miss_key) continue;
+        if (end[j] != 0u && end[j] - p <= kShortRun) {
+            runs[slot] = make_uint2(p, end[j]);
+            run_row[slot] = row[j];
+            ++slot;
+        } else {
+            lruns[lslot++] = make_uint2(p, end[j]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t r = t; r < total; r += kAlbBlock) {
+        const uint2 run = runs[r];
+        // added in order, four loads in flight at a time
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        for (uint32_t i = run.x; i < run.y; i += 4u) {
+            Rgb v[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k)
+                if (i + k < run.y) v[k] = vals_sorted[i + k];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k)
+                if (i + k < run.y) {
+                    sr += v[k].r;
+                    sg += v[k].g;
+                    sb += v[k].b;
+                }
+        }
+        const size_t g = (size_t)run_row[r] * 3;
+        grad[g + 0] += sr;
+        grad[g + 1] += sg;
+        grad[g + 2] += sb;
+    }
+    sum_long_runs(keys_sorted, vals_sorted, n, lruns, total_long, grad);
+}
+
 // Device buffers of the deterministic reduction for up to cap_px pixels.
 struct DetWorkspace {
     uint32_t *keys, *keys_sorted; // row keys: written in pixel order, sorted in place of each other
     uint32_t *pix, *pix_sorted;   // the pixel of each key
     Rgb* vals;                    // contributions, by pixel
     Rgb* vals_sorted;             // contributions in sorted order
-    uint2* long_runs;             // [first, end) of each run longer than kShortRun, kLongPerWindow slots per window
-    uint32_t* long_count;         // slots used per window
     uint32_t* count;              // the sort's digit counts per tile (kRadix x tiles), then its kRadix digit totals
     size_t cap_px;
 };
@@ -781,14 +775,12 @@ static vx_status grow(vx_albedo_state* a, size_t px)
     DetWorkspace& w = a->ws;
     if (px <= w.cap_px) return VX_OK;
     for (void* p : {(void*)w.keys, (void*)w.keys_sorted, (void*)w.pix, (void*)w.pix_sorted, (void*)w.vals,
-                    (void*)w.vals_sorted, (void*)w.long_runs, (void*)w.long_count, (void*)w.count})
+                    (void*)w.vals_sorted, (void*)w.count})
         (void)hipFree(p);
     w = DetWorkspace{};
     if (hipMalloc(&w.keys, px * 4) != hipSuccess || hipMalloc(&w.keys_sorted, px * 4) != hipSuccess ||
         hipMalloc(&w.pix, px * 4) != hipSuccess || hipMalloc(&w.pix_sorted, px * 4) != hipSuccess ||
         hipMalloc(&w.vals, px * sizeof(Rgb)) != hipSuccess || hipMalloc(&w.vals_sorted, px * sizeof(Rgb)) != hipSuccess ||
-        hipMalloc(&w.long_runs, run_windows(px) * kLongPerWindow * sizeof(uint2)) != hipSuccess ||
-        hipMalloc(&w.long_count, run_windows(px) * 4) != hipSuccess ||
         hipMalloc(&w.count, count_words(px) * 4) != hipSuccess)
         return fail(VX_ERR_OUT_OF_MEMORY, "albedo workspace allocation failed");
     w.cap_px = px;
@@ -823,8 +815,6 @@ static vx_status scratch_workspace(int device, size_t px, DetWorkspace& w)
     w.pix_sorted = static_cast<uint32_t*>(take(px * 4));
     w.vals = static_cast<Rgb*>(take(px * sizeof(Rgb)));
     w.vals_sorted = static_cast<Rgb*>(take(px * sizeof(Rgb)));
-    w.long_runs = static_cast<uint2*>(take(run_windows(px) * kLongPerWindow * sizeof(uint2)));
-    w.long_count = static_cast<uint32_t*>(take(run_windows(px) * 4));
     w.count = static_cast<uint32_t*>(take(count_words(px) * 4));
     w.cap_px = px;
     if (off > kScratchBytes)
@@ -841,7 +831,7 @@ void vx_albedo_state_free(vx_albedo_state* a)
     if (!a) return;
     const vx::DetWorkspace& w = a->ws;
     for (void* p : {(void*)a->base, (void*)w.keys, (void*)w.keys_sorted, (void*)w.pix, (void*)w.pix_sorted,
-                    (void*)w.vals, (void*)w.vals_sorted, (void*)w.long_runs, (void*)w.long_count, (void*)w.count})
+                    (void*)w.vals, (void*)w.vals_sorted, (void*)w.count})
         (void)hipFree(p);
     delete a;
 }
@@ -971,10 +961,7 @@ static vx_status albedo_backward(vx_scene* s, const vx_camera* cam, uint32_t w, 
     if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": sort failed");
     // the sorted keys are in kin, their contributions in vals_sorted; one workgroup per window of positions
     const uint32_t n_windows = (uint32_t)run_windows(px);
-    k_reduce_runs<<<dim3(n_windows), block, 0, st>>>(kin, ws.vals_sorted, n, miss_key, grad_albedo, ws.long_runs,
-                                                     ws.long_count);
-    k_reduce_long_runs<<<dim3(n_windows), block, 0, st>>>(kin, ws.vals_sorted, n, ws.long_runs, ws.long_count,
-                                                          grad_albedo);
+    k_reduce_runs<<<dim3(n_windows), block, 0, st>>>(kin, ws.vals_sorted, n, miss_key, grad_albedo);
     if (hipGetLastError() != hipSuccess) return fail(VX_ERR_HIP, std::string(where) + ": run reduction failed");
     return VX_OK;
 }
