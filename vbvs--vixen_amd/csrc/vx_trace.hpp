@@ -98,6 +98,30 @@ __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
 
+// Scene arrays always live in device global memory. Loads go through a
+// global-address-space pointer, so they compile to global loads (base in SGPRs,
+// vector memory counter only) even where the compiler cannot see where a
+// pointer came from: the batched kernel reads its scene views from a table in
+// memory, and with generic pointers it emitted flat loads, which also count
+// against the LDS wait counter and so made every wait on the traversal's LDS
+// columns wait for outstanding memory loads too.
+template <class T>
+using gmem_ptr = const __attribute__((address_space(1))) T*;
+
+// A 4- or 8-byte scene element loaded as a plain integer through a global
+// pointer (a vector type's copy constructor would take the address back as a
+// generic reference and the load would become a flat load again).
+template <class T>
+__device__ __forceinline__ T ld_global(const char* p)
+{
+    if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, *(gmem_ptr<unsigned long long>)p);
+    } else {
+        static_assert(sizeof(T) == 4, "4- or 8-byte scene elements");
+        return __builtin_bit_cast(T, *(gmem_ptr<uint32_t>)p);
+    }
+}
+
 // Element i of a scene array through a 32-bit byte offset from its (uniform)
 // base, so the load takes the base in SGPRs and only a 32-bit offset per lane,
 // with no 64-bit address arithmetic. Valid while i * sizeof(T) < 2^32: the
@@ -106,13 +130,13 @@ __device__ __forceinline__ uint32_t popc8(uint32_t v) { return __builtin_popcoun
 template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i)
 {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+    return ld_global<T>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
 }
 // The same with a byte offset.
 template <class T>
 __device__ __forceinline__ T ldb32(const T* __restrict__ base, uint32_t byte_offset)
 {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_offset);
+    return ld_global<T>(reinterpret_cast<const char*>(base) + byte_offset);
 }
 
 __device__ __forceinline__ void store_miss(vx_hit& h, uint32_t iters)
@@ -186,10 +210,10 @@ __device__ __forceinline__ void rgba8_decode(uint32_t w, float rgb[3])
 // (shaders/VoxelRayMarch.comp:189-191) as a 256-entry palette.
 __device__ __forceinline__ void palette_decode(const float* __restrict__ palette, uint32_t w, float rgb[3])
 {
-    const float* c = palette + (w & 255u) * 3u;
-    rgb[0] = c[0];
-    rgb[1] = c[1];
-    rgb[2] = c[2];
+    const uint32_t c = (w & 255u) * 3u;
+    rgb[0] = ld32(palette, c);
+    rgb[1] = ld32(palette, c + 1u);
+    rgb[2] = ld32(palette, c + 2u);
 }
 
 // Ray set-up in the grid's local voxel space [0,N]^3. Returns false on a miss
@@ -459,8 +483,8 @@ __device__ __forceinline__ void rec_load(const SceneView& s, vx_hit& h)
     h.voxel = *lane_word(s, s.aux_base, 4);
     if (f & VX_HIT_FLAG_LOD) h.rgb[0] = h.rgb[1] = h.rgb[2] = 0.0f;
     else if (s.bc1) bc1_decode(ld32(s.bc1, h.brick * 32u + (h.voxel >> 4)), h.voxel & 15u, h.rgb);
-    else if (s.palette) palette_decode(s.palette, s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
-    else rgba8_decode(s.brick_data[(size_t)h.brick * 512u + h.voxel], h.rgb);
+    else if (s.palette) palette_decode(s.palette, ld_global<uint32_t>(reinterpret_cast<const char*>(s.brick_data + (size_t)h.brick * 512u + h.voxel)), h.rgb);
+    else rgba8_decode(ld_global<uint32_t>(reinterpret_cast<const char*>(s.brick_data + (size_t)h.brick * 512u + h.voxel)), h.rgb);
 }
 
 // Per-lane LDS words after the stack: the brick-DDA inputs ol[3], dl[3],
