@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kMaxGroup) __attribute__((amdgpu_waves_per_eu(
         for (uint32_t s = 0; s < n_scenes; ++s) {
             const uint2* src = scenes[s].view.nodes;
             const uint32_t off = scenes[s].view.lds_off, cnt = scenes[s].view.lds_count;
-            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) vx_lds_nodes[off + i] = src[i];
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) vx_lds_nodes[off + i] = ld32(src, i);
         }
         __syncthreads();
     }
