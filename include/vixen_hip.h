@@ -371,8 +371,9 @@ vx_status vx_batch_destroy(vx_batch* batch);
 vx_status vx_trace_batched(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
                            const vx_tuning* tuning, vx_stream stream);
 
-/* Times every shape (waves {1,2,4,8} x chunk {32..1024} x LDS depth {0..3})
- * on these rays, keeps the fastest for later calls and reports it. */
+/* Times every shape (waves {1,2,4,8} x chunk {32..1024} x LDS depth {0..3},
+ * refill at 32 idle lanes) on these rays, then refill thresholds 8, 16 and 24
+ * for the fastest shape; keeps the fastest for later calls and reports it. */
 vx_status vx_batch_autotune(vx_batch* batch, const vx_ray* rays, const uint64_t* ray_offsets, vx_hit* hits,
                             vx_tuning* best, float* best_ms, vx_stream stream);
 

@@ -411,30 +411,53 @@ static vx_status autotune(vx_batch* b, const vx_ray* rays, const uint64_t* off, 
     vx_tuning bt = default_tuning();
     float bms = 1e30f;
     vx_status status = VX_OK;
+    // the fastest of three timed launches after one untimed, for one shape
+    auto time_shape = [&](const vx_tuning& t, bool& skip) {
+        float ms = 1e30f;
+        skip = false;
+        std::vector<BatchSceneInfo> tab;
+        uint64_t n_chunks = 0;
+        uint32_t lds_bytes = 0;
+        bool staged = false;
+        if ((status = make_table(b, off, t, tab, n_chunks, lds_bytes, staged)) != VX_OK) return ms;
+        if (t.lds_depth > 0 && !staged) { // nothing staged: same as depth 0
+            skip = true;
+            return ms;
+        }
+        for (int rep = 0; rep < 4 && status == VX_OK; ++rep) {
+            if (hipEventRecord(e0, st) != hipSuccess) status = fail(VX_ERR_HIP, "event record failed");
+            if (status == VX_OK) status = trace_batched(b, rays, off, hits, &t, st);
+            if (status == VX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                status = fail(VX_ERR_HIP, "event sync failed");
+            float m = 0.0f;
+            if (status == VX_OK && hipEventElapsedTime(&m, e0, e1) == hipSuccess && rep > 0) ms = std::min(ms, m);
+        }
+        return ms;
+    };
     for (uint32_t depth = 0; depth <= 3 && status == VX_OK; ++depth)
         for (uint32_t w : waves)
             for (uint32_t c : chunks) {
                 const vx_tuning t{w, c, depth, 32};
-                std::vector<BatchSceneInfo> tab;
-                uint64_t n_chunks = 0;
-                uint32_t lds_bytes = 0;
-                bool staged = false;
-                if ((status = make_table(b, off, t, tab, n_chunks, lds_bytes, staged)) != VX_OK) break;
-                if (depth > 0 && !staged) continue; // nothing staged: same as depth 0
-                float ms = 1e30f;
-                for (int rep = 0; rep < 3 && status == VX_OK; ++rep) {
-                    if (hipEventRecord(e0, st) != hipSuccess) status = fail(VX_ERR_HIP, "event record failed");
-                    if (status == VX_OK) status = trace_batched(b, rays, off, hits, &t, st);
-                    if (status == VX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
-                        status = fail(VX_ERR_HIP, "event sync failed");
-                    float m = 0.0f;
-                    if (status == VX_OK && hipEventElapsedTime(&m, e0, e1) == hipSuccess && rep > 0) ms = std::min(ms, m);
-                }
-                if (status == VX_OK && ms < bms) {
+                bool skip = false;
+                const float ms = time_shape(t, skip);
+                if (status != VX_OK) break;
+                if (!skip && ms < bms) {
                     bms = ms;
                     bt = t;
                 }
             }
+    // then the refill threshold for the best shape (fewer idle lanes before a refill
+    // suits sets of long incoherent rays)
+    for (uint32_t rm : {8u, 16u, 24u}) {
+        if (status != VX_OK) break;
+        const vx_tuning t{bt.waves_per_group, bt.chunk, bt.lds_depth, rm};
+        bool skip = false;
+        const float ms = time_shape(t, skip);
+        if (status == VX_OK && !skip && ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (status != VX_OK) return status;
