@@ -176,13 +176,22 @@ public:
 private:
     void trace_host(const vx_ray* rays, size_t n, const LODParameters* lod, vx_hit* out) const;
 
+    // Staging of the host-memory casts: two slots, each with device ray and hit
+    // buffers, page-locked host copies of both and a stream, so that one chunk's
+    // transfers and trace overlap the host-side copies of the next.
+    struct Slot {
+        vx_ray* d_rays = nullptr;
+        vx_hit* d_hits = nullptr;
+        vx_ray* h_rays = nullptr;
+        vx_hit* h_hits = nullptr;
+        void* stream = nullptr;
+    };
+
     vx_ctx* ctx_ = nullptr;
     vx_scene* scene_ = nullptr;
     int device_ = 0;
-    mutable std::mutex mu_;       // the staging buffers serve one cast at a time
-    mutable vx_ray* d_rays_ = nullptr;
-    mutable vx_hit* d_hits_ = nullptr;
-    mutable void* stream_ = nullptr;
+    mutable std::mutex mu_;       // the staging slots serve one cast at a time
+    mutable Slot slots_[2];
 };
 
 } // namespace vx::svo

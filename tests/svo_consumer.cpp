@@ -10,10 +10,15 @@
 //                                 castRayScreenSpaceLOD, castRayWithLOD and
 //                                 castRays on the seeded 128^3 cave with BC1
 //                                 (seed 5, 50% occupancy)
+//   svo_consumer rate N           GPU 0: castRays of N seeded rays from host
+//                                 memory on the 256^3 cave (seed 2, 50%
+//                                 occupancy), median of 5; prints one JSON line
 //
 // Each ray group is written as u64 n, n vx_ray, n records of 80 bytes:
 // entity u64, hitPoint 3f, normal 3f, tMin f, tMax f, scale i32, hit u32,
 // brick u32, voxel u32, leafNode u32, color 3f, iterations u32, lod u32.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -221,6 +226,29 @@ int trace(const char* out)
     return 0;
 }
 
+int rate(size_t n)
+{
+    Built b;
+    build_cave(256, 2, 0.5, true, b);
+    const LaineKarrasOctreeHip oct(0, b.desc());
+    Rng rng{77};
+    const std::vector<vx_ray> rays = seeded_rays(rng, n, 256.0f);
+    (void)oct.castRays(rays.data(), rays.size()); // warm-up: staging buffers, first launches
+    std::vector<double> ms;
+    size_t hits = 0;
+    for (int k = 0; k < 5; ++k) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const std::vector<RayHit> h = oct.castRays(rays.data(), rays.size());
+        ms.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        hits = 0;
+        for (const RayHit& x : h) hits += x.hit ? 1u : 0u;
+    }
+    std::sort(ms.begin(), ms.end());
+    std::printf("{\"rays\": %zu, \"castRays_ms\": %.3f, \"mrays\": %.1f, \"hits\": %zu}\n", n, ms[2],
+                (double)n / ms[2] / 1e3, hits);
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -230,7 +258,8 @@ int main(int argc, char** argv)
         if (mode == "queries" && argc == 3) return queries(argv[2]);
         if (mode == "convert" && argc == 4) return convert(argv[2], argv[3]);
         if (mode == "trace" && argc == 3) return trace(argv[2]);
-        std::fprintf(stderr, "usage: svo_consumer queries OUT | convert IN OUT | trace OUT\n");
+        if (mode == "rate" && argc == 3) return rate(std::stoul(argv[2]));
+        std::fprintf(stderr, "usage: svo_consumer queries OUT | convert IN OUT | trace OUT | rate N\n");
         return 2;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "svo_consumer: %s\n", e.what());

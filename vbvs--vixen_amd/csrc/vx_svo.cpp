@@ -10,6 +10,8 @@
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
+#include <system_error>
+#include <thread>
 
 namespace vx::svo {
 
@@ -255,9 +257,16 @@ LaineKarrasOctreeHip::~LaineKarrasOctreeHip()
     // no exceptions here: the device is switched by hand, and every call's status is dropped
     int prev = -1;
     const bool switched = hipGetDevice(&prev) == hipSuccess && prev != device_ && hipSetDevice(device_) == hipSuccess;
-    if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
-    (void)hipFree(d_rays_);
-    (void)hipFree(d_hits_);
+    for (Slot& sl : slots_) {
+        if (sl.stream) {
+            (void)hipStreamSynchronize(static_cast<hipStream_t>(sl.stream));
+            (void)hipStreamDestroy(static_cast<hipStream_t>(sl.stream));
+        }
+        (void)hipFree(sl.d_rays);
+        (void)hipFree(sl.d_hits);
+        (void)hipHostFree(sl.h_rays);
+        (void)hipHostFree(sl.h_hits);
+    }
     if (switched) (void)hipSetDevice(prev);
     (void)vx_scene_destroy(scene_);
     (void)vx_ctx_destroy(ctx_);
@@ -277,21 +286,44 @@ void LaineKarrasOctreeHip::trace_host(const vx_ray* rays, size_t n, const LODPar
 {
     std::lock_guard<std::mutex> lock(mu_);
     OnDevice on(device_);
-    if (!stream_) {
-        hipStream_t st = nullptr;
-        check_hip(hipStreamCreate(&st), "hipStreamCreate");
-        stream_ = st;
+    for (Slot& sl : slots_) {
+        if (!sl.stream) {
+            hipStream_t st = nullptr;
+            check_hip(hipStreamCreate(&st), "hipStreamCreate");
+            sl.stream = st;
+        }
+        // a cast that threw may have left copies in flight on the slot
+        check_hip(hipStreamSynchronize(static_cast<hipStream_t>(sl.stream)), "hipStreamSynchronize");
+        if (!sl.d_rays) check_hip(hipMalloc(reinterpret_cast<void**>(&sl.d_rays), kStage * sizeof(vx_ray)), "hipMalloc");
+        if (!sl.d_hits) check_hip(hipMalloc(reinterpret_cast<void**>(&sl.d_hits), kStage * sizeof(vx_hit)), "hipMalloc");
+        if (!sl.h_rays)
+            check_hip(hipHostMalloc(reinterpret_cast<void**>(&sl.h_rays), kStage * sizeof(vx_ray), hipHostMallocDefault),
+                      "hipHostMalloc");
+        if (!sl.h_hits)
+            check_hip(hipHostMalloc(reinterpret_cast<void**>(&sl.h_hits), kStage * sizeof(vx_hit), hipHostMallocDefault),
+                      "hipHostMalloc");
     }
-    if (!d_rays_) check_hip(hipMalloc(reinterpret_cast<void**>(&d_rays_), kStage * sizeof(vx_ray)), "hipMalloc");
-    if (!d_hits_) check_hip(hipMalloc(reinterpret_cast<void**>(&d_hits_), kStage * sizeof(vx_hit)), "hipMalloc");
-    auto st = static_cast<hipStream_t>(stream_);
-    for (size_t at = 0; at < n; at += kStage) {
-        const size_t m = std::min(kStage, n - at);
-        check_hip(hipMemcpyAsync(d_rays_, rays + at, m * sizeof(vx_ray), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-        check(castRaysDevice(d_rays_, m, d_hits_, st, lod), "castRays");
-        check_hip(hipMemcpyAsync(out + at, d_hits_, m * sizeof(vx_hit), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    // chunk c uses slot c % 2: its rays are copied into the slot's page-locked buffer, sent, traced and sent
+    // back on the slot's stream while the host fills the other slot; a slot's records are copied out before
+    // the slot takes its next chunk
+    const size_t chunks = (n + kStage - 1) / kStage;
+    auto retire = [&](size_t c) {
+        const Slot& sl = slots_[c & 1u];
+        const size_t at = c * kStage, m = std::min(kStage, n - at);
+        check_hip(hipStreamSynchronize(static_cast<hipStream_t>(sl.stream)), "hipStreamSynchronize");
+        std::memcpy(out + at, sl.h_hits, m * sizeof(vx_hit));
+    };
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c >= 2) retire(c - 2);
+        const Slot& sl = slots_[c & 1u];
+        const auto st = static_cast<hipStream_t>(sl.stream);
+        const size_t at = c * kStage, m = std::min(kStage, n - at);
+        std::memcpy(sl.h_rays, rays + at, m * sizeof(vx_ray));
+        check_hip(hipMemcpyAsync(sl.d_rays, sl.h_rays, m * sizeof(vx_ray), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check(castRaysDevice(sl.d_rays, m, sl.d_hits, st, lod), "castRays");
+        check_hip(hipMemcpyAsync(sl.h_hits, sl.d_hits, m * sizeof(vx_hit), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     }
+    for (size_t c = chunks >= 2 ? chunks - 2 : 0; c < chunks; ++c) retire(c);
 }
 
 std::vector<RayHit> LaineKarrasOctreeHip::castRays(const vx_ray* rays, size_t n, const LODParameters* lod) const
@@ -299,7 +331,24 @@ std::vector<RayHit> LaineKarrasOctreeHip::castRays(const vx_ray* rays, size_t n,
     std::vector<vx_hit> h(n);
     if (n) trace_host(rays, n, lod, h.data());
     std::vector<RayHit> out(n);
-    for (size_t i = 0; i < n; ++i) out[i] = toRayHit(rays[i], h[i]);
+    // the RayHit conversion over up to 16 host threads for large batches (a range whose thread cannot be
+    // started runs inline)
+    const auto convert = [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) out[i] = toRayHit(rays[i], h[i]);
+    };
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = n < (size_t{1} << 16) ? 1 : std::min<size_t>(std::min<size_t>(hw, 16), n >> 14);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t b = n * t / nt, e = n * (t + 1) / nt;
+        try {
+            pool.emplace_back(convert, b, e);
+        } catch (const std::system_error&) {
+            convert(b, e);
+        }
+    }
+    convert(0, n / nt);
+    for (std::thread& th : pool) th.join();
     return out;
 }
 
