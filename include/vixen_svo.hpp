@@ -164,7 +164,9 @@ public:
                           float tMin = 0.0f, float tMax = 1e30f) const;
 
     /* Many rays at once from host memory (castRay, or castRayWithLOD when lod is
-     * given), in chunks through the adapter's device staging buffers. */
+     * given), in 64 Ki-ray chunks through the adapter's two page-locked staging
+     * slots (one chunk's transfers and trace overlap the host copies of the
+     * next); large batches convert their records on up to 16 host threads. */
     std::vector<RayHit> castRays(const vx_ray* rays, size_t n, const LODParameters* lod = nullptr) const;
     /* The same on device buffers (vx_trace / vx_trace_lod on the caller's
      * stream; asynchronous, no copies). */
